@@ -1,0 +1,184 @@
+"""Adjoint-method gradients of <psi(theta)|H|psi(theta)> on the GPU.
+
+Thin wrapper over the ``qs_pcircuit_*`` / ``qs_adjoint_*`` C ABI:
+
+* :class:`ParamCircuit` -- fixed gates (k <= 2) and rotations
+  exp(-i theta_p sigma/2) about X, Y or Z, with layer marks.  Rotation
+  conventions follow the OpenQASM U gate of the reference
+  (src/openqasm.rs:145-163): RY(t) = u(t, 0, 0), RZ(t) = u(0, 0, t).
+* :class:`Expectation` -- ``forward(thetas)`` returns the batch of energies
+  and keeps the final states on the device; ``backward()`` runs one reverse
+  sweep and returns dE/dtheta for the whole batch (the Function-style
+  forward/backward pair).
+* :class:`ExpectationFunction` -- a ``torch.autograd.Function`` over the
+  same pair, so the engine plugs into PyTorch-ROCm autograd.
+
+H is a sum of Pauli-Z strings, given as (masks, coeffs): term t contributes
+coeffs[t] * (-1)^popcount(x & masks[t]) on basis state x.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .register import _check, _dtype_code, _sizes, _vp
+
+AXIS = {"x": 0, "y": 1, "z": 2}
+
+
+class ParamCircuit:
+    def __init__(self, n: int):
+        h = ctypes.c_void_p()
+        _check(_lib.load().qs_pcircuit_new(int(n), ctypes.byref(h)))
+        self._h = h
+        self.n = int(n)
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _lib.load().qs_pcircuit_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gate(self, matrix, targets: Sequence[int]) -> "ParamCircuit":
+        m = np.ascontiguousarray(matrix, dtype=np.complex128)
+        t = _sizes(targets)
+        _check(_lib.load().qs_pcircuit_gate(self._h, _vp(m), _vp(t), t.size))
+        return self
+
+    def rotation(self, axis: str, target: int, param: int) -> "ParamCircuit":
+        _check(_lib.load().qs_pcircuit_rotation(self._h, AXIS[axis.lower()], int(target),
+                                                int(param)))
+        return self
+
+    def layer(self) -> "ParamCircuit":
+        _check(_lib.load().qs_pcircuit_layer(self._h))
+        return self
+
+    @property
+    def nparams(self) -> int:
+        return int(_lib.load().qs_pcircuit_nparams(self._h))
+
+    @staticmethod
+    def from_ansatz(ansatz) -> "ParamCircuit":
+        """Build from :class:`quaru_amd.circuits.Ansatz`."""
+        from .operation import cz
+        pc = ParamCircuit(ansatz.n)
+        ends = set(ansatz.layer_ends)
+        for i, op in enumerate(ansatz.ops):
+            if op[0] in ("rx", "ry", "rz"):
+                pc.rotation(op[0][1], op[1], op[2])
+            elif op[0] == "cz":
+                g = cz(op[1], op[2])
+                pc.gate(g.matrix(), g.targets())
+            else:
+                raise ValueError(f"unknown ansatz op {op[0]}")
+            if i + 1 in ends:
+                pc.layer()
+        return pc
+
+
+class Expectation:
+    """Energies and adjoint gradients of a batch of parameter sets."""
+
+    def __init__(self, circuit: ParamCircuit, masks, coeffs, dtype="c128",
+                 checkpoint_every: int = 0):
+        self.circuit = circuit
+        self.masks = np.ascontiguousarray(np.asarray(masks, dtype=np.uint64).reshape(-1))
+        self.coeffs = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64).reshape(-1))
+        if self.masks.size != self.coeffs.size:
+            raise ValueError("masks and coeffs differ in length")
+        self.dtype = _dtype_code(dtype)
+        self.checkpoint_every = int(checkpoint_every)
+        self._ctx = None
+        self._batch = 0
+
+    def _free(self):
+        if self._ctx is not None and self._ctx.value:
+            _lib.load().qs_adjoint_free(self._ctx)
+        self._ctx = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+    def _thetas(self, thetas) -> np.ndarray:
+        th = np.ascontiguousarray(np.asarray(thetas, dtype=np.float64))
+        if th.ndim == 1:
+            th = th.reshape(1, -1)
+        if th.shape[1] != self.circuit.nparams:
+            raise ValueError(f"expected {self.circuit.nparams} parameters, got {th.shape[1]}")
+        return np.ascontiguousarray(th)
+
+    def energies(self, thetas) -> np.ndarray:
+        """Forward only; nothing is kept on the device."""
+        th = self._thetas(thetas)
+        out = np.zeros(th.shape[0], dtype=np.float64)
+        _check(_lib.load().qs_adjoint_forward(self.circuit._h, _vp(th), th.shape[0],
+                                              _vp(self.masks), _vp(self.coeffs), self.masks.size,
+                                              self.dtype, 0, None, _vp(out)))
+        return out
+
+    def forward(self, thetas) -> np.ndarray:
+        self._free()
+        th = self._thetas(thetas)
+        out = np.zeros(th.shape[0], dtype=np.float64)
+        ctx = ctypes.c_void_p()
+        _check(_lib.load().qs_adjoint_forward(self.circuit._h, _vp(th), th.shape[0],
+                                              _vp(self.masks), _vp(self.coeffs), self.masks.size,
+                                              self.dtype, self.checkpoint_every,
+                                              ctypes.byref(ctx), _vp(out)))
+        self._ctx = ctx
+        self._batch = th.shape[0]
+        return out
+
+    def backward(self) -> np.ndarray:
+        if self._ctx is None:
+            raise RuntimeError("backward() needs a forward() first")
+        g = np.zeros((self._batch, self.circuit.nparams), dtype=np.float64)
+        _check(_lib.load().qs_adjoint_backward(self._ctx, _vp(g)))
+        self._free()
+        return g
+
+
+def _torch_function():
+    import torch
+
+    class ExpectationFunction(torch.autograd.Function):
+        """``E = ExpectationFunction.apply(thetas, expectation)``: thetas is a
+        (batch, nparams) float64 tensor; E has shape (batch,)."""
+
+        @staticmethod
+        def forward(ctx, thetas, expectation: Expectation):
+            th = thetas.detach().to("cpu", torch.float64).numpy()
+            e = expectation.forward(th)
+            ctx.expectation = expectation
+            ctx.src_device = thetas.device
+            return torch.from_numpy(e).to(thetas.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            g = torch.from_numpy(ctx.expectation.backward()).to(ctx.src_device)
+            return grad_out.to(g.dtype).unsqueeze(1) * g, None
+
+    return ExpectationFunction
+
+
+_EF = None
+
+
+def expectation_function():
+    """The torch.autograd.Function class (imports torch on first use)."""
+    global _EF
+    if _EF is None:
+        _EF = _torch_function()
+    return _EF

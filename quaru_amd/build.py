@@ -1,0 +1,80 @@
+"""Build the engine's shared library for gfx950 with hipcc.
+
+``python -m quaru_amd.build`` (or ``build()``) compiles every HIP source in
+``quaru_amd/csrc`` into ``quaru_amd/libquaru_hip.so``.  hipcc cross-compiles
+for gfx950 without a GPU present.  Packed FP32 instructions are disabled
+(``-packed-fp32-ops``) for every HIP translation unit.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libquaru_hip.so")
+SOURCES = ["qs_kernels.hip", "qs_sweep.hip", "qs_register.hip", "qs_adjoint.hip"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the engine needs ROCm's hipcc to build")
+
+
+def build(verbose: bool = False, jobs: int = 4) -> str:
+    """Compile each source to an object in parallel, then link the library."""
+    cc = hipcc()
+    objdir = tempfile.mkdtemp(prefix="quaru_hip_build_")
+    try:
+        procs = []
+        objs = []
+        for src in SOURCES:
+            path = os.path.join(CSRC, src)
+            if not os.path.exists(path):
+                continue
+            obj = os.path.join(objdir, src + ".o")
+            objs.append(obj)
+            cmd = [cc, *HIPCC_FLAGS, "-c", path, "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.STDOUT, text=True)))
+            while len([p for _, p in procs if p.poll() is None]) >= jobs:
+                procs[0][1].wait()
+        failed = []
+        for src, p in procs:
+            out, _ = p.communicate()
+            if p.returncode != 0:
+                failed.append((src, out))
+            elif verbose and out.strip():
+                print(out)
+        if failed:
+            msg = "\n".join(f"--- {s}\n{o}" for s, o in failed)
+            raise RuntimeError(f"hipcc failed:\n{msg}")
+        tmp = LIB + ".tmp"
+        cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, LIB)
+    finally:
+        shutil.rmtree(objdir, ignore_errors=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

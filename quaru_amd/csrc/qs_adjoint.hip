@@ -1,0 +1,617 @@
+// qs_adjoint.hip -- adjoint-method gradients of <psi(theta)|H|psi(theta)>.
+//
+// The reference has no differentiation (quaru 0.2.0 only has the OpenQASM U
+// gate as a parameterised operation, src/openqasm.rs:145-163, whose
+// conventions the rotations here follow: RY(t) = u(t, 0, 0), RZ(t) =
+// u(0, 0, t)).
+//
+// Batch layout: B parameter sets are padded to Bp = 2^nb states stored back
+// to back, [Bp][2^n]; the batch index is then nb extra high index bits, so a
+// fixed gate on qubit q < n is one launch of the ordinary gate kernels over
+// n + nb "qubits".  Rotations carry one matrix per batch element.
+//
+// Forward:  psi_b = U_N ... U_1 |0>,  E_b = sum_x |psi_b(x)|^2 h(x) with
+//           h(x) = sum_t c_t (-1)^popcount(x & mask_t).
+// Backward (one reverse sweep, no per-parameter reruns):
+//           lambda = h . psi; for i = N..1:
+//             rotation i on parameter p:  g_p += Im <lambda| sigma |psi>
+//               (= 2 Re <lambda| dU_i/dtheta U_i^dagger |psi>, with
+//                dU/dtheta = -(i/2) sigma U),
+//             then psi <- U_i^dagger psi, lambda <- U_i^dagger lambda
+//           (the three steps are one fused pass over psi and lambda).
+// Checkpoints: with checkpoint_every = L > 0 the forward keeps psi after
+// every L-th layer, and the backward re-anchors psi to that copy when it
+// reaches it, bounding the rounding drift of the uncompute (and, in
+// complex64, keeping the reverse sweep within the forward's accuracy).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/quaru_hip.h"
+#include "qs_internal.h"
+
+namespace {
+
+template <typename R>
+struct AV2;
+template <>
+struct AV2<double> {
+  using T = double2;
+};
+template <>
+struct AV2<float> {
+  using T = float2;
+};
+
+constexpr int kBlock = 256;
+constexpr int kItems = 4;
+
+struct POp {
+  int kind = 0;  // 0 fixed gate, 1 rotation
+  int axis = 0;
+  int target = 0;
+  size_t param = 0;
+  qs::Gate fwd, inv;
+  int rot_slot = -1;
+};
+
+int fail_arg(const char *msg) { return qs::set_error(QS_ERR_ARGUMENT, 0, 0, msg); }
+
+__device__ __forceinline__ uint64_t insert0(uint64_t p, int q) {
+  const uint64_t lo = p & ((1ull << q) - 1ull);
+  return ((p ^ lo) << 1) | lo;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_init_zero_states(typename AV2<R>::T *psi, int n,
+                                                             uint64_t total) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * kBlock) {
+    typename AV2<R>::T v;
+    v.x = ((i & ((1ull << n) - 1ull)) == 0) ? (R)1 : (R)0;
+    v.y = 0;
+    psi[i] = v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_htable(double *h, int n,
+                                                   const unsigned long long *__restrict__ masks,
+                                                   const double *__restrict__ coeffs,
+                                                   int nterms) {
+  const uint64_t len = 1ull << n;
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < len;
+       x += (uint64_t)gridDim.x * kBlock) {
+    double v = 0.0;
+    for (int t = 0; t < nterms; ++t)
+      v += (__popcll((unsigned long long)x & masks[t]) & 1) ? -coeffs[t] : coeffs[t];
+    h[x] = v;
+  }
+}
+
+// Rotation with one 2x2 matrix per batch element (mats[b*4 + e], complex).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_rot(typename AV2<R>::T *__restrict__ psi, int n,
+                                                int q, const typename AV2<R>::T *__restrict__ mats,
+                                                uint64_t pairs_per_state, uint64_t total_pairs) {
+  using T = typename AV2<R>::T;
+  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kItems) + threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t w = base + (uint64_t)it * kBlock;
+    if (w >= total_pairs) break;
+    const uint64_t b = w / pairs_per_state, p = w % pairs_per_state;
+    const uint64_t off = b << n;
+    const uint64_t i0 = off | insert0(p, q), i1 = i0 | (1ull << q);
+    const T m0 = mats[4 * b], m1 = mats[4 * b + 1], m2 = mats[4 * b + 2], m3 = mats[4 * b + 3];
+    const T x = psi[i0], y = psi[i1];
+    T u, v;
+    u.x = m0.x * x.x - m0.y * x.y + m1.x * y.x - m1.y * y.y;
+    u.y = m0.x * x.y + m0.y * x.x + m1.x * y.y + m1.y * y.x;
+    v.x = m2.x * x.x - m2.y * x.y + m3.x * y.x - m3.y * y.y;
+    v.y = m2.x * x.y + m2.y * x.x + m3.x * y.y + m3.y * y.x;
+    psi[i0] = u;
+    psi[i1] = v;
+  }
+}
+
+// One reverse step of a rotation: accumulate Im <lam|sigma|psi> per batch
+// element into part[b * nblk + blockIdx.x], then apply U^dagger to psi and
+// lam.  grid = (nblk, Bp); block x covers pairs [x*per, (x+1)*per) of state b.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_rot_reverse(typename AV2<R>::T *__restrict__ psi,
+                                                        typename AV2<R>::T *__restrict__ lam,
+                                                        int n, int q, int axis,
+                                                        const typename AV2<R>::T *__restrict__ mats,
+                                                        uint64_t pairs_per_state, uint64_t per,
+                                                        double *__restrict__ part) {
+  using T = typename AV2<R>::T;
+  __shared__ double sh[kBlock];
+  const uint64_t b = blockIdx.y;
+  const uint64_t off = b << n;
+  const T m0 = mats[4 * b], m1 = mats[4 * b + 1], m2 = mats[4 * b + 2], m3 = mats[4 * b + 3];
+  // U^dagger = [[conj m0, conj m2], [conj m1, conj m3]]
+  double acc = 0.0;
+  const uint64_t p0 = (uint64_t)blockIdx.x * per;
+  uint64_t p1 = p0 + per;
+  if (p1 > pairs_per_state) p1 = pairs_per_state;
+  for (uint64_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
+    const uint64_t i0 = off | insert0(p, q), i1 = i0 | (1ull << q);
+    const T x0 = psi[i0], x1 = psi[i1];
+    const T l0 = lam[i0], l1 = lam[i1];
+    // s = sigma . psi
+    double s0r, s0i, s1r, s1i;
+    if (axis == 0) {
+      s0r = x1.x; s0i = x1.y; s1r = x0.x; s1i = x0.y;
+    } else if (axis == 1) {
+      s0r = x1.y; s0i = -(double)x1.x; s1r = -(double)x0.y; s1i = x0.x;
+    } else {
+      s0r = x0.x; s0i = x0.y; s1r = -(double)x1.x; s1i = -(double)x1.y;
+    }
+    // Im(conj(l0) s0 + conj(l1) s1)
+    acc += (double)l0.x * s0i - (double)l0.y * s0r + (double)l1.x * s1i - (double)l1.y * s1r;
+    T y0, y1, k0, k1;
+    y0.x = m0.x * x0.x + m0.y * x0.y + m2.x * x1.x + m2.y * x1.y;
+    y0.y = m0.x * x0.y - m0.y * x0.x + m2.x * x1.y - m2.y * x1.x;
+    y1.x = m1.x * x0.x + m1.y * x0.y + m3.x * x1.x + m3.y * x1.y;
+    y1.y = m1.x * x0.y - m1.y * x0.x + m3.x * x1.y - m3.y * x1.x;
+    k0.x = m0.x * l0.x + m0.y * l0.y + m2.x * l1.x + m2.y * l1.y;
+    k0.y = m0.x * l0.y - m0.y * l0.x + m2.x * l1.y - m2.y * l1.x;
+    k1.x = m1.x * l0.x + m1.y * l0.y + m3.x * l1.x + m3.y * l1.y;
+    k1.y = m1.x * l0.y - m1.y * l0.x + m3.x * l1.y - m3.y * l1.x;
+    psi[i0] = y0;
+    psi[i1] = y1;
+    lam[i0] = k0;
+    lam[i1] = k1;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// grad[b * P + param] += sum_k part[b * nblk + k]   (grid = Bp blocks)
+__global__ __launch_bounds__(kBlock) void k_grad_accum(const double *__restrict__ part, int nblk,
+                                                       double *__restrict__ grad, size_t P,
+                                                       size_t param) {
+  __shared__ double sh[kBlock];
+  const int b = blockIdx.x;
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += kBlock) v += part[(size_t)b * nblk + k];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) grad[(size_t)b * P + param] += sh[0];
+}
+
+// part[b * nblk + x] = sum over the block's chunk of |psi_b|^2 h
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_expect(const typename AV2<R>::T *__restrict__ psi,
+                                                   int n, const double *__restrict__ h,
+                                                   uint64_t per, double *__restrict__ part) {
+  __shared__ double sh[kBlock];
+  const uint64_t b = blockIdx.y;
+  const uint64_t len = 1ull << n;
+  const uint64_t x0 = (uint64_t)blockIdx.x * per;
+  uint64_t x1 = x0 + per;
+  if (x1 > len) x1 = len;
+  double acc = 0.0;
+  for (uint64_t x = x0 + threadIdx.x; x < x1; x += kBlock) {
+    const auto a = psi[(b << n) | x];
+    acc += ((double)a.x * (double)a.x + (double)a.y * (double)a.y) * h[x];
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_rows(const double *__restrict__ part, int nblk,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[kBlock];
+  const int b = blockIdx.x;
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += kBlock) v += part[(size_t)b * nblk + k];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = sh[0];
+}
+
+// lam = h . psi
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_apply_h(typename AV2<R>::T *__restrict__ lam,
+                                                    const typename AV2<R>::T *__restrict__ psi,
+                                                    int n, const double *__restrict__ h,
+                                                    uint64_t total) {
+  const uint64_t mask = (1ull << n) - 1ull;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * kBlock) {
+    auto a = psi[i];
+    const double hv = h[i & mask];
+    a.x = (R)((double)a.x * hv);
+    a.y = (R)((double)a.y * hv);
+    lam[i] = a;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(double *p, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * kBlock)
+    p[i] = 0.0;
+}
+
+unsigned grid_of(uint64_t work, uint64_t per, unsigned cap = 262144) {
+  uint64_t g = (work + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+struct qs_pcircuit {
+  int n = 0;
+  std::vector<POp> ops;
+  std::vector<size_t> layer_ends;
+  size_t nparams = 0;
+  int nrot = 0;
+};
+
+struct qs_adjoint {
+  qs_pcircuit circ;
+  int n = 0, nb = 0;
+  size_t B = 0, Bp = 1, P = 0;
+  int dtype = QS_C128;
+  size_t checkpoint_every = 0;
+  hipStream_t stream = nullptr;
+  void *psi = nullptr, *lam = nullptr;
+  std::vector<void *> ckpt;
+  std::vector<size_t> ckpt_at;  // checkpoint j holds psi after ops [0, ckpt_at[j])
+  double *d_h = nullptr;
+  void *d_rot = nullptr;        // [nrot][Bp][4] complex, precision of dtype
+  void *d_gmat = nullptr;       // diagonal tables of fixed gates (forward, inverse)
+  std::vector<size_t> gmat_off; // per op: offset of fwd table, inv table (bytes)
+  double *d_part = nullptr;
+  size_t part_count = 0;
+  double *d_grad = nullptr;
+  size_t amp_bytes() const { return dtype == QS_C128 ? 16 : 8; }
+  size_t state_bytes() const { return amp_bytes() * (Bp << n); }
+};
+
+namespace {
+
+void free_ctx(qs_adjoint *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->psi) (void)hipFree(c->psi);
+  if (c->lam) (void)hipFree(c->lam);
+  for (void *p : c->ckpt) (void)hipFree(p);
+  if (c->d_h) (void)hipFree(c->d_h);
+  if (c->d_rot) (void)hipFree(c->d_rot);
+  if (c->d_gmat) (void)hipFree(c->d_gmat);
+  if (c->d_part) (void)hipFree(c->d_part);
+  if (c->d_grad) (void)hipFree(c->d_grad);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// Apply a classified fixed gate over all Bp states (n + nb index bits).
+template <typename R>
+hipError_t apply_fixed(qs_adjoint *c, void *buf, const qs::Gate &g, const void *d_table) {
+  const int ntot = c->n + c->nb;
+  switch (g.kind) {
+    case qs::Gate::kIdentity: return hipSuccess;
+    case qs::Gate::kSparse: return qs::launch_gate_sparse<R>(buf, ntot, g, c->stream);
+    case qs::Gate::kDiag:
+      return qs::launch_gate_diag<R>(buf, ntot, g, const_cast<void *>(d_table), c->stream);
+    case qs::Gate::kDense:
+      return qs::launch_gate_dense<R>(buf, ntot, g, nullptr, nullptr, c->stream);
+  }
+  return hipSuccess;
+}
+
+template <typename R>
+int forward_t(qs_adjoint *c, const double *thetas, const uint64_t *masks, const double *coeffs,
+              size_t nterms, double *energies) {
+  using T = typename AV2<R>::T;
+  const int n = c->n;
+  const size_t Bp = c->Bp, B = c->B, P = c->P;
+  const uint64_t total = (uint64_t)Bp << n;
+  hipError_t e = hipSuccess;
+#define AJ(expr)                                          \
+  do {                                                    \
+    e = (expr);                                           \
+    if (e != hipSuccess) return qs::set_hip_error(e, #expr); \
+  } while (0)
+  AJ(hipMalloc(&c->psi, c->state_bytes()));
+  // observable table
+  {
+    unsigned long long *d_m = nullptr;
+    double *d_c = nullptr;
+    AJ(hipMalloc((void **)&c->d_h, sizeof(double) << n));
+    AJ(hipMalloc((void **)&d_m, sizeof(unsigned long long) * (nterms ? nterms : 1)));
+    AJ(hipMalloc((void **)&d_c, sizeof(double) * (nterms ? nterms : 1)));
+    if (nterms) {
+      AJ(hipMemcpy(d_m, masks, sizeof(unsigned long long) * nterms, hipMemcpyHostToDevice));
+      AJ(hipMemcpy(d_c, coeffs, sizeof(double) * nterms, hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(k_htable, dim3(grid_of(1ull << n, kBlock * 4)), dim3(kBlock), 0, c->stream,
+                       c->d_h, n, d_m, d_c, (int)nterms);
+    AJ(hipGetLastError());
+    AJ(hipStreamSynchronize(c->stream));
+    (void)hipFree(d_m);
+    (void)hipFree(d_c);
+  }
+  // rotation matrices, one per (rotation, batch element)
+  const int nrot = c->circ.nrot;
+  std::vector<R> rot((size_t)std::max(nrot, 1) * Bp * 8, (R)0);
+  for (const POp &op : c->circ.ops) {
+    if (op.kind != 1) continue;
+    for (size_t b = 0; b < Bp; ++b) {
+      const double th = b < B ? thetas[b * P + op.param] : 0.0;
+      const double cs = std::cos(th / 2.0), sn = std::sin(th / 2.0);
+      double m[8];
+      if (op.axis == QS_AXIS_X) {
+        const double v[8] = {cs, 0, 0, -sn, 0, -sn, cs, 0};
+        std::memcpy(m, v, sizeof(m));
+      } else if (op.axis == QS_AXIS_Y) {
+        const double v[8] = {cs, 0, -sn, 0, sn, 0, cs, 0};
+        std::memcpy(m, v, sizeof(m));
+      } else {
+        const double v[8] = {cs, -sn, 0, 0, 0, 0, cs, sn};
+        std::memcpy(m, v, sizeof(m));
+      }
+      R *dst = rot.data() + ((size_t)op.rot_slot * Bp + b) * 8;
+      for (int k = 0; k < 8; ++k) dst[k] = (R)m[k];
+    }
+  }
+  AJ(hipMalloc(&c->d_rot, rot.size() * sizeof(R)));
+  AJ(hipMemcpy(c->d_rot, rot.data(), rot.size() * sizeof(R), hipMemcpyHostToDevice));
+  // diagonal tables of the fixed gates
+  {
+    std::vector<R> tab;
+    c->gmat_off.assign(c->circ.ops.size() * 2, 0);
+    for (size_t i = 0; i < c->circ.ops.size(); ++i) {
+      const POp &op = c->circ.ops[i];
+      if (op.kind != 0) continue;
+      const qs::Gate *gs[2] = {&op.fwd, &op.inv};
+      for (int s = 0; s < 2; ++s) {
+        c->gmat_off[2 * i + s] = tab.size() * sizeof(R);
+        if (gs[s]->kind == qs::Gate::kDiag)
+          for (const qs::cd &v : gs[s]->m) {
+            tab.push_back((R)v.re);
+            tab.push_back((R)v.im);
+          }
+      }
+    }
+    if (tab.empty()) tab.push_back((R)0);
+    AJ(hipMalloc(&c->d_gmat, tab.size() * sizeof(R)));
+    AJ(hipMemcpy(c->d_gmat, tab.data(), tab.size() * sizeof(R), hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(k_init_zero_states<R>, dim3(grid_of(total, kBlock * 4)), dim3(kBlock), 0,
+                     c->stream, (T *)c->psi, n, total);
+  AJ(hipGetLastError());
+  const uint64_t pps = 1ull << (n - 1);
+  size_t layer = 0;
+  for (size_t i = 0; i < c->circ.ops.size(); ++i) {
+    const POp &op = c->circ.ops[i];
+    if (op.kind == 1) {
+      const T *mats = (const T *)c->d_rot + (size_t)op.rot_slot * Bp * 4;
+      const uint64_t tp = pps * Bp;
+      hipLaunchKernelGGL(k_rot<R>, dim3(grid_of(tp, kBlock * kItems, 0xffffffffu)), dim3(kBlock),
+                         0, c->stream, (T *)c->psi, n, op.target, mats, pps, tp);
+      AJ(hipGetLastError());
+    } else {
+      AJ(apply_fixed<R>(c, c->psi, op.fwd, (const char *)c->d_gmat + c->gmat_off[2 * i]));
+    }
+    // checkpoint after every checkpoint_every-th layer (not after the last)
+    while (layer < c->circ.layer_ends.size() && c->circ.layer_ends[layer] == i + 1) {
+      ++layer;
+      if (c->checkpoint_every > 0 && layer % c->checkpoint_every == 0 &&
+          layer < c->circ.layer_ends.size()) {
+        void *cp = nullptr;
+        AJ(hipMalloc(&cp, c->state_bytes()));
+        AJ(hipMemcpyAsync(cp, c->psi, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream));
+        c->ckpt.push_back(cp);
+        c->ckpt_at.push_back(i + 1);
+      }
+    }
+  }
+  // energies
+  const uint64_t len = 1ull << n;
+  const uint64_t per = std::min<uint64_t>(len, 1ull << 14);
+  const int nblk = (int)((len + per - 1) / per);
+  c->part_count = std::max<size_t>((size_t)nblk, (size_t)((pps + 1023) / 1024)) * Bp;
+  AJ(hipMalloc((void **)&c->d_part, c->part_count * sizeof(double)));
+  double *d_e = nullptr;
+  AJ(hipMalloc((void **)&d_e, Bp * sizeof(double)));
+  hipLaunchKernelGGL(k_expect<R>, dim3(nblk, (unsigned)Bp), dim3(kBlock), 0, c->stream,
+                     (const T *)c->psi, n, c->d_h, per, c->d_part);
+  hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)Bp), dim3(kBlock), 0, c->stream,
+                     (const double *)c->d_part, nblk, d_e);
+  AJ(hipGetLastError());
+  std::vector<double> eh(Bp);
+  AJ(hipMemcpyAsync(eh.data(), d_e, Bp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  AJ(hipStreamSynchronize(c->stream));
+  (void)hipFree(d_e);
+  if (energies)
+    for (size_t b = 0; b < B; ++b) energies[b] = eh[b];
+  return QS_OK;
+}
+
+template <typename R>
+int backward_t(qs_adjoint *c, double *grads) {
+  using T = typename AV2<R>::T;
+  const int n = c->n;
+  const size_t Bp = c->Bp, P = c->P;
+  const uint64_t total = (uint64_t)Bp << n;
+  hipError_t e = hipSuccess;
+  if (!c->lam) AJ(hipMalloc(&c->lam, c->state_bytes()));
+  if (!c->d_grad) AJ(hipMalloc((void **)&c->d_grad, Bp * std::max<size_t>(P, 1) * sizeof(double)));
+  hipLaunchKernelGGL(k_fill_zero, dim3(grid_of(Bp * P, kBlock)), dim3(kBlock), 0, c->stream,
+                     c->d_grad, (uint64_t)(Bp * P));
+  hipLaunchKernelGGL(k_apply_h<R>, dim3(grid_of(total, kBlock * 4)), dim3(kBlock), 0, c->stream,
+                     (T *)c->lam, (const T *)c->psi, n, c->d_h, total);
+  AJ(hipGetLastError());
+  const uint64_t pps = 1ull << (n - 1);
+  const uint64_t per = std::min<uint64_t>(pps, 1024);
+  const int nblk = (int)((pps + per - 1) / per);
+  int ck = (int)c->ckpt.size() - 1;
+  for (size_t ii = c->circ.ops.size(); ii-- > 0;) {
+    if (ck >= 0 && c->ckpt_at[ck] == ii + 1) {
+      AJ(hipMemcpyAsync(c->psi, c->ckpt[ck], c->state_bytes(), hipMemcpyDeviceToDevice,
+                        c->stream));
+      --ck;
+    }
+    const POp &op = c->circ.ops[ii];
+    if (op.kind == 1) {
+      const T *mats = (const T *)c->d_rot + (size_t)op.rot_slot * Bp * 4;
+      hipLaunchKernelGGL(k_rot_reverse<R>, dim3(nblk, (unsigned)Bp), dim3(kBlock), 0, c->stream,
+                         (T *)c->psi, (T *)c->lam, n, op.target, op.axis, mats, pps, per,
+                         c->d_part);
+      hipLaunchKernelGGL(k_grad_accum, dim3((unsigned)Bp), dim3(kBlock), 0, c->stream,
+                         (const double *)c->d_part, nblk, c->d_grad, P, op.param);
+      AJ(hipGetLastError());
+    } else {
+      const void *tab = (const char *)c->d_gmat + c->gmat_off[2 * ii + 1];
+      AJ(apply_fixed<R>(c, c->psi, op.inv, tab));
+      AJ(apply_fixed<R>(c, c->lam, op.inv, tab));
+    }
+  }
+  std::vector<double> g(Bp * P);
+  AJ(hipMemcpyAsync(g.data(), c->d_grad, g.size() * sizeof(double), hipMemcpyDeviceToHost,
+                    c->stream));
+  AJ(hipStreamSynchronize(c->stream));
+  std::memcpy(grads, g.data(), c->B * P * sizeof(double));
+  return QS_OK;
+#undef AJ
+}
+
+}  // namespace
+
+extern "C" {
+
+int qs_pcircuit_new(size_t n, qs_pcircuit **out) {
+  if (!out) return fail_arg("out is NULL");
+  if (n < 1 || n > 30) return qs::set_error(QS_ERR_ARGUMENT, n, 0, "pcircuit needs 1..30 qubits");
+  qs_pcircuit *c = new qs_pcircuit();
+  c->n = (int)n;
+  *out = c;
+  return QS_OK;
+}
+
+void qs_pcircuit_free(qs_pcircuit *c) { delete c; }
+
+int qs_pcircuit_gate(qs_pcircuit *c, const double *matrix, const size_t *targets, size_t k) {
+  if (!c || !matrix || !targets) return fail_arg("NULL argument");
+  if (k < 1 || k > 2) return qs::set_error(QS_INVALID_DIMENSIONS, k, k, "fixed gates: k = 1 or 2");
+  std::vector<size_t> t(targets, targets + k);
+  if (k == 2 && t[0] == t[1]) return qs::set_error(QS_INVALID_TARGET, t[0], 0, "duplicate target");
+  for (size_t j = 0; j < k; ++j)
+    if (t[j] >= (size_t)c->n) return qs::set_error(QS_INVALID_TARGET, t[j], 0, "target out of range");
+  const size_t D = (size_t)1 << k;
+  std::vector<double> inv(2 * D * D);
+  for (size_t i = 0; i < D; ++i)
+    for (size_t j = 0; j < D; ++j) {
+      inv[2 * (i * D + j)] = matrix[2 * (j * D + i)];
+      inv[2 * (i * D + j) + 1] = -matrix[2 * (j * D + i) + 1];
+    }
+  POp op;
+  op.kind = 0;
+  op.fwd = qs::classify_matrix(matrix, t.data(), k, 0, 0);
+  op.inv = qs::classify_matrix(inv.data(), t.data(), k, 0, 0);
+  if ((op.fwd.kind == qs::Gate::kDense && op.fwd.na > 2) ||
+      (op.fwd.kind == qs::Gate::kDiag && op.fwd.na > 2))
+    return qs::set_error(QS_INVALID_DIMENSIONS, k, k, "unsupported fixed gate");
+  c->ops.push_back(std::move(op));
+  return QS_OK;
+}
+
+int qs_pcircuit_rotation(qs_pcircuit *c, int axis, size_t target, size_t param) {
+  if (!c) return fail_arg("NULL argument");
+  if (axis < QS_AXIS_X || axis > QS_AXIS_Z) return fail_arg("axis must be 0 (X), 1 (Y) or 2 (Z)");
+  if (target >= (size_t)c->n) return qs::set_error(QS_INVALID_TARGET, target, 0, "target out of range");
+  POp op;
+  op.kind = 1;
+  op.axis = axis;
+  op.target = (int)target;
+  op.param = param;
+  op.rot_slot = c->nrot++;
+  c->nparams = std::max(c->nparams, param + 1);
+  c->ops.push_back(std::move(op));
+  return QS_OK;
+}
+
+int qs_pcircuit_layer(qs_pcircuit *c) {
+  if (!c) return fail_arg("NULL argument");
+  c->layer_ends.push_back(c->ops.size());
+  return QS_OK;
+}
+
+size_t qs_pcircuit_nparams(const qs_pcircuit *c) { return c ? c->nparams : 0; }
+
+int qs_adjoint_forward(const qs_pcircuit *c, const double *thetas, size_t batch,
+                       const uint64_t *masks, const double *coeffs, size_t nterms, int dtype,
+                       size_t checkpoint_every, qs_adjoint **ctx, double *energies) {
+  if (ctx) *ctx = nullptr;
+  if (!c || batch == 0 || (c->nparams > 0 && !thetas) || (nterms > 0 && (!masks || !coeffs)))
+    return fail_arg("NULL argument or empty batch");
+  if (dtype != QS_C128 && dtype != QS_C64) return fail_arg("bad dtype");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return qs::set_error(QS_ERR_NO_DEVICE, 0, 0, "no HIP device available");
+  qs_adjoint *a = new qs_adjoint();
+  a->circ = *c;
+  a->n = c->n;
+  a->B = batch;
+  a->Bp = 1;
+  a->nb = 0;
+  while (a->Bp < batch) {
+    a->Bp <<= 1;
+    ++a->nb;
+  }
+  if (a->n + a->nb > qs::kMaxQubits) {
+    delete a;
+    return fail_arg("batch x state too large");
+  }
+  a->P = c->nparams;
+  a->dtype = dtype;
+  a->checkpoint_every = checkpoint_every;
+  hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete a;
+    return qs::set_hip_error(e, "hipStreamCreate");
+  }
+  const int rc = dtype == QS_C128 ? forward_t<double>(a, thetas, masks, coeffs, nterms, energies)
+                                  : forward_t<float>(a, thetas, masks, coeffs, nterms, energies);
+  if (rc != QS_OK || !ctx) {
+    free_ctx(a);
+    return rc;
+  }
+  *ctx = a;
+  return QS_OK;
+}
+
+int qs_adjoint_backward(qs_adjoint *ctx, double *grads) {
+  if (!ctx || !grads) return fail_arg("NULL argument");
+  if (ctx->P == 0) return QS_OK;
+  return ctx->dtype == QS_C128 ? backward_t<double>(ctx, grads) : backward_t<float>(ctx, grads);
+}
+
+void qs_adjoint_free(qs_adjoint *ctx) { free_ctx(ctx); }
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// qs_internal.h -- types shared by the engine's translation units.
+//
+// The state vector is 2^n amplitudes, {re, im} interleaved, in basis-index
+// order, qubit q = bit q (src/register.rs:35, :41-50).  complex128 registers
+// store double2, complex64 registers float2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace qs {
+
+constexpr int kMaxQubits = 40;
+constexpr int kMaxFixed = 64;  // bit positions a kernel can hold fixed
+
+struct cd {
+  double re, im;
+};
+
+// A gate after classification on the host.
+//  kDense : 2^na x 2^na matrix `m` on qubits act[0..na), where
+//           (idx & cmask) == cval.
+//  kDiag  : 2^na diagonal entries `m` on act[], same condition; na may be 0
+//           (then `m[0]` multiplies every selected amplitude).
+//  kSparse: a diagonal whose entries are 1 except the few listed in
+//           (spat, m): amplitude idx is multiplied by m[e] when
+//           (idx & smask) == spat[e]; smask covers act[] and the controls.
+struct Gate {
+  enum Kind : int { kIdentity = 0, kDense = 1, kDiag = 2, kSparse = 3 };
+  Kind kind = kIdentity;
+  uint64_t cmask = 0, cval = 0;
+  int na = 0;
+  int act[kMaxFixed] = {};
+  std::vector<cd> m;
+  uint64_t smask = 0;
+  std::vector<uint64_t> spat;
+  // Number of logical gates this entry stands for (apply_all counts n).
+  int logical = 1;
+};
+
+// Bit positions held fixed while the kernel enumerates the free index bits:
+// the free counter p is expanded by inserting a zero at each pos[] (sorted
+// ascending), then `set` is OR-ed in.
+struct Fixed {
+  uint64_t set;
+  int nf;
+  uint8_t pos[kMaxFixed];
+};
+
+Fixed make_fixed(uint64_t mask, uint64_t set);
+
+// Classify a dense 2^k x 2^k complex128 matrix on `targets` (qs_register.hip);
+// cmask/cval are controls already known.
+Gate classify_matrix(const double *mat, const size_t *targets, size_t k, uint64_t cmask,
+                     uint64_t cval);
+
+// Record the thread's last error (read back by qs_last_error); returns code.
+int set_error(int code, size_t a, size_t b, const std::string &msg);
+int set_hip_error(hipError_t e, const char *where);
+
+// ---- kernels (qs_kernels.hip) ---------------------------------------------
+// All launches go to `s`; `psi` is the device state of 2^n amplitudes.
+template <typename R>
+hipError_t launch_init_basis(void *psi, int n, uint64_t k, hipStream_t s);
+template <typename R>
+hipError_t launch_init_product(void *psi, int n, const double *d_qubits, hipStream_t s);
+template <typename R>
+hipError_t launch_gate_dense(void *psi, int n, const Gate &g, void *d_mat, void *scratch,
+                             hipStream_t s);
+template <typename R>
+hipError_t launch_gate_diag(void *psi, int n, const Gate &g, void *d_mat, hipStream_t s);
+template <typename R>
+hipError_t launch_gate_sparse(void *psi, int n, const Gate &g, hipStream_t s);
+
+// Reductions write FP64 results to device memory `d_out` (caller copies).
+// d_part must hold kReduceBlocks * 2 doubles.
+constexpr int kReduceBlocks = 1024;
+template <typename R>
+hipError_t launch_prob_split(const void *psi, int n, int target, double *d_part, double *d_out,
+                             hipStream_t s);
+template <typename R>
+hipError_t launch_collapse(void *psi, int n, int target, int keep, double total, hipStream_t s);
+template <typename R>
+hipError_t launch_norm(const void *psi, int n, double *d_part, double *d_out, hipStream_t s);
+hipError_t launch_inner(const void *a, bool a_c64, const void *b, bool b_c64, int n,
+                        double *d_part, double *d_out, hipStream_t s);
+template <typename R>
+hipError_t launch_neq_count(const void *a, const void *b, int n, double tol, double *d_part,
+                            double *d_out, hipStream_t s);
+template <typename R>
+hipError_t launch_probabilities(const void *psi, int n, double *d_out, hipStream_t s);
+template <typename R>
+hipError_t launch_convert_from_c128(void *psi, const double *src, uint64_t count, hipStream_t s);
+template <typename R>
+hipError_t launch_convert_to_c128(double *dst, const void *psi, uint64_t count, hipStream_t s);
+// Histogram of |psi|^2 over the bit pattern of m qubits: bin bit j <->
+// qubit pos_order[j]; fx holds those m positions fixed (set = 0).  d_part
+// holds blocks x 2^m partials.
+template <typename R>
+hipError_t launch_set_histogram(const void *psi, int n, const Fixed &fx, const uint8_t *pos_order,
+                                int m, double *d_part, int blocks, double *d_hist, hipStream_t s);
+// |psi|^2 of the amplitudes expand(j, fx), j < count, into d_out.
+template <typename R>
+hipError_t launch_gather_probs(const void *psi, const Fixed &fx, uint64_t count, double *d_out,
+                               hipStream_t s);
+// Leave only amplitude idx, divided by sqrt(total); zero the rest.
+template <typename R>
+hipError_t launch_collapse_to(void *psi, int n, uint64_t idx, double total, hipStream_t s);
+// Chunk sums of |psi|^2 for the CDF: chunk c covers [c*chunk, (c+1)*chunk).
+template <typename R>
+hipError_t launch_chunk_sums(const void *psi, int n, uint64_t chunk, double *d_out,
+                             hipStream_t s);
+
+// ---- fused sweep engine (qs_sweep.hip) --------------------------------------
+struct SweepPlan;  // defined in qs_sweep.h
+}  // namespace qs

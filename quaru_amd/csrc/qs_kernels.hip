@@ -1,0 +1,857 @@
+// qs_kernels.hip -- per-gate kernels and FP64 reductions for gfx950.
+//
+// The per-gate kernels replace the hot loop of Register::try_apply_fast
+// (src/register.rs:155-189): instead of filtering all 2^n indices and
+// gathering each group through heap vectors, a thread expands a dense counter
+// p over the free index bits into a group base index (inserting zeros at the
+// sorted fixed bit positions: targets and controls), then reads, multiplies
+// and writes the group in place.  Groups are disjoint, so in-place equals the
+// reference's out-of-place copy.  Amplitudes whose control bits do not match
+// are never touched, which is why a gate with c controls moves 2^(n-c)
+// amplitudes instead of 2^n.
+//
+// Reductions (measure, norm, inner product, equality) accumulate in FP64 for
+// both storage precisions, with a fixed grid so results are deterministic.
+#include "qs_internal.h"
+
+namespace qs {
+
+template <typename R>
+struct V2;
+template <>
+struct V2<double> {
+  using T = double2;
+};
+template <>
+struct V2<float> {
+  using T = float2;
+};
+
+Fixed make_fixed(uint64_t mask, uint64_t set) {
+  Fixed f{};
+  f.set = set;
+  f.nf = 0;
+  for (int b = 0; b < 64; ++b)
+    if ((mask >> b) & 1ull) f.pos[f.nf++] = (uint8_t)b;
+  return f;
+}
+
+__device__ __forceinline__ uint64_t expand(uint64_t p, const Fixed &f) {
+  for (int i = 0; i < f.nf; ++i) {
+    const uint64_t lo = p & ((1ull << f.pos[i]) - 1ull);
+    p = ((p ^ lo) << 1) | lo;
+  }
+  return p | f.set;
+}
+
+template <typename T>
+__device__ __forceinline__ T cmla(T acc, decltype(T::x) mr, decltype(T::x) mi, T a) {
+  acc.x = fma(mr, a.x, acc.x);
+  acc.x = fma(-mi, a.y, acc.x);
+  acc.y = fma(mr, a.y, acc.y);
+  acc.y = fma(mi, a.x, acc.y);
+  return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ T czero() {
+  T z;
+  z.x = 0;
+  z.y = 0;
+  return z;
+}
+
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(uint64_t work, uint64_t per_block) {
+  uint64_t g = (work + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+// ---- state initialisation ---------------------------------------------------
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_init_basis(typename V2<R>::T *psi, uint64_t len,
+                                                       uint64_t k) {
+  using T = typename V2<R>::T;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    T v = czero<T>();
+    if (i == k) v.x = 1;
+    psi[i] = v;
+  }
+}
+
+// Product state in the order of the kron fold of Register::new /
+// try_from_qubits (src/register.rs:47-50, :94-96): amp = b_{n-1} * (... *
+// (b_0 * 1)).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_init_product(typename V2<R>::T *psi, int n,
+                                                         const double *__restrict__ q) {
+  using T = typename V2<R>::T;
+  const uint64_t len = 1ull << n;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    double ar = 1.0, ai = 0.0;
+    for (int b = 0; b < n; ++b) {
+      const int bit = (int)((i >> b) & 1ull);
+      const double br = q[4 * b + 2 * bit], bi = q[4 * b + 2 * bit + 1];
+      const double nr = br * ar - bi * ai;
+      const double ni = br * ai + bi * ar;
+      ar = nr;
+      ai = ni;
+    }
+    T v;
+    v.x = (R)ar;
+    v.y = (R)ai;
+    psi[i] = v;
+  }
+}
+
+template <typename R>
+hipError_t launch_init_basis(void *psi, int n, uint64_t k, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  unsigned g = grid_for(len, kBlock * 8);
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_init_basis<R>, dim3(g), dim3(kBlock), 0, s,
+                     (typename V2<R>::T *)psi, len, k);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_init_product(void *psi, int n, const double *d_qubits, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  unsigned g = grid_for(len, kBlock * 8);
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_init_product<R>, dim3(g), dim3(kBlock), 0, s,
+                     (typename V2<R>::T *)psi, n, d_qubits);
+  return hipGetLastError();
+}
+
+// ---- dense gates --------------------------------------------------------------
+
+template <typename R, int D>
+struct MatArg {
+  R re[D * D];
+  R im[D * D];
+};
+
+// One active target (matrix bit 0 <-> qubit with stride `tbit`).  Each thread
+// owns kItems pairs spaced one block apart, so a wave issues all its loads
+// before the first multiply.
+template <typename R, int kItems>
+__global__ __launch_bounds__(kBlock) void k_gate1(typename V2<R>::T *__restrict__ psi,
+                                                  uint64_t npairs, Fixed fx, uint64_t tbit,
+                                                  MatArg<R, 2> m) {
+  using T = typename V2<R>::T;
+  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kItems) + threadIdx.x;
+  T a0[kItems], a1[kItems];
+  uint64_t i0[kItems];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t p = base + (uint64_t)it * kBlock;
+    if (p < npairs) {
+      i0[it] = expand(p, fx);
+      a0[it] = psi[i0[it]];
+      a1[it] = psi[i0[it] | tbit];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t p = base + (uint64_t)it * kBlock;
+    if (p < npairs) {
+      T r0 = czero<T>(), r1 = czero<T>();
+      r0 = cmla(r0, m.re[0], m.im[0], a0[it]);
+      r0 = cmla(r0, m.re[1], m.im[1], a1[it]);
+      r1 = cmla(r1, m.re[2], m.im[2], a0[it]);
+      r1 = cmla(r1, m.re[3], m.im[3], a1[it]);
+      psi[i0[it]] = r0;
+      psi[i0[it] | tbit] = r1;
+    }
+  }
+}
+
+// Two active targets: quadruples (b, b|t0, b|t1, b|t0|t1) and a 4x4 matrix.
+template <typename R, int kItems>
+__global__ __launch_bounds__(kBlock) void k_gate2(typename V2<R>::T *__restrict__ psi,
+                                                  uint64_t nquads, Fixed fx, uint64_t t0,
+                                                  uint64_t t1, MatArg<R, 4> m) {
+  using T = typename V2<R>::T;
+  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kItems) + threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t p = base + (uint64_t)it * kBlock;
+    if (p >= nquads) break;
+    const uint64_t b = expand(p, fx);
+    const uint64_t idx[4] = {b, b | t0, b | t1, b | t0 | t1};
+    T a[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = psi[idx[j]];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      T r = czero<T>();
+#pragma unroll
+      for (int j = 0; j < 4; ++j) r = cmla(r, m.re[4 * i + j], m.im[4 * i + j], a[j]);
+      psi[idx[i]] = r;
+    }
+  }
+}
+
+// K = 3..5 active targets.  A block stages kSlots amplitudes -- G = kSlots /
+// 2^K groups -- in LDS.  Slot s holds row i = s / G of group g = s % G, so
+// consecutive lanes read consecutive groups (consecutive free bits) from HBM
+// and LDS reads of a column j are bank-conflict free.  The matrix is staged
+// transposed (mt[j * D + i] = M[i][j]) so a wave's reads of one column are a
+// broadcast.  Every slot is read before any is written: in place is safe.
+template <typename R, int K>
+__global__ __launch_bounds__(kBlock) void k_gatek(typename V2<R>::T *__restrict__ psi,
+                                                  uint64_t ngroups, Fixed fx, Fixed ax,
+                                                  const typename V2<R>::T *__restrict__ mt) {
+  using T = typename V2<R>::T;
+  constexpr int D = 1 << K;
+  constexpr int kSlots = 1024;
+  constexpr int G = kSlots / D;
+  constexpr int kPer = kSlots / kBlock;
+  __shared__ T sa[kSlots];
+  __shared__ T sm[D * D];
+  for (int e = threadIdx.x; e < D * D; e += kBlock) sm[e] = mt[e];
+  const uint64_t g0 = (uint64_t)blockIdx.x * G;
+  uint64_t gidx[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int s = threadIdx.x + r * kBlock;
+    const int g = s % G, i = s / G;
+    gidx[r] = ~0ull;
+    if (g0 + g < ngroups) {
+      // offset of row i: deposit the K bits of i at the active positions
+      uint64_t off = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        if ((i >> j) & 1) off |= 1ull << ax.pos[j];
+      gidx[r] = expand(g0 + g, fx) | off;
+      sa[s] = psi[gidx[r]];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int s = threadIdx.x + r * kBlock;
+    const int g = s % G, i = s / G;
+    if (gidx[r] == ~0ull) continue;
+    T acc = czero<T>();
+#pragma unroll 8
+    for (int j = 0; j < D; ++j) {
+      const T mv = sm[j * D + i];
+      acc = cmla(acc, mv.x, mv.y, sa[j * G + g]);
+    }
+    psi[gidx[r]] = acc;
+  }
+}
+
+// Any K: out-of-place from `src` (a copy of the state) into psi, one thread
+// per (group, row).  Used for the dense k = n operations of the examples.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_gate_big(typename V2<R>::T *__restrict__ psi,
+                                                     const typename V2<R>::T *__restrict__ src,
+                                                     uint64_t ngroups, int K, Fixed fx, Fixed ax,
+                                                     const typename V2<R>::T *__restrict__ m) {
+  using T = typename V2<R>::T;
+  const uint64_t D = 1ull << K;
+  const uint64_t total = ngroups * D;
+  for (uint64_t w = (uint64_t)blockIdx.x * kBlock + threadIdx.x; w < total;
+       w += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t i = w % D, g = w / D;
+    const uint64_t b = expand(g, fx);
+    uint64_t oi = 0;
+    for (int j = 0; j < K; ++j)
+      if ((i >> j) & 1ull) oi |= 1ull << ax.pos[j];
+    T acc = czero<T>();
+    for (uint64_t j = 0; j < D; ++j) {
+      uint64_t oj = 0;
+      for (int q = 0; q < K; ++q)
+        if ((j >> q) & 1ull) oj |= 1ull << ax.pos[q];
+      const T mv = m[i * D + j];
+      acc = cmla(acc, mv.x, mv.y, src[b | oj]);
+    }
+    psi[b | oi] = acc;
+  }
+}
+
+template <typename R, int D>
+static MatArg<R, D> to_matarg(const Gate &g) {
+  MatArg<R, D> m;
+  for (int e = 0; e < D * D; ++e) {
+    m.re[e] = (R)g.m[e].re;
+    m.im[e] = (R)g.m[e].im;
+  }
+  return m;
+}
+
+// Active positions in matrix-bit order (not sorted): pos[j] = act[j].
+static Fixed active_positions(const Gate &g) {
+  Fixed a{};
+  a.nf = g.na;
+  a.set = 0;
+  for (int j = 0; j < g.na; ++j) a.pos[j] = (uint8_t)g.act[j];
+  return a;
+}
+
+static uint64_t active_mask(const Gate &g) {
+  uint64_t m = 0;
+  for (int j = 0; j < g.na; ++j) m |= 1ull << g.act[j];
+  return m;
+}
+
+// d_mat layout: K in 3..5 -> transposed D x D; K > 5 -> row major D x D;
+// values in precision R, interleaved.  `scratch` (2^n amplitudes) is needed
+// only when K > 5.
+template <typename R>
+hipError_t launch_gate_dense(void *psi_v, int n, const Gate &g, void *d_mat, void *scratch,
+                             hipStream_t s) {
+  using T = typename V2<R>::T;
+  T *psi = (T *)psi_v;
+  const uint64_t amask = active_mask(g);
+  const Fixed fx = make_fixed(g.cmask | amask, g.cval);
+  const int nfree = n - fx.nf;
+  const uint64_t nfreeN = 1ull << nfree;
+  if (g.na == 1) {
+    constexpr int kItems = 4;
+    hipLaunchKernelGGL((k_gate1<R, kItems>), dim3(grid_for(nfreeN, kBlock * kItems)),
+                       dim3(kBlock), 0, s, psi, nfreeN, fx, 1ull << g.act[0],
+                       to_matarg<R, 2>(g));
+  } else if (g.na == 2) {
+    constexpr int kItems = 2;
+    hipLaunchKernelGGL((k_gate2<R, kItems>), dim3(grid_for(nfreeN, kBlock * kItems)),
+                       dim3(kBlock), 0, s, psi, nfreeN, fx, 1ull << g.act[0], 1ull << g.act[1],
+                       to_matarg<R, 4>(g));
+  } else if (g.na >= 3 && g.na <= 5) {
+    const Fixed ax = active_positions(g);
+    const T *mt = (const T *)d_mat;
+    const unsigned grid = grid_for(nfreeN, 1024 >> g.na);
+    if (g.na == 3)
+      hipLaunchKernelGGL((k_gatek<R, 3>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, mt);
+    else if (g.na == 4)
+      hipLaunchKernelGGL((k_gatek<R, 4>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, mt);
+    else
+      hipLaunchKernelGGL((k_gatek<R, 5>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, mt);
+  } else {
+    const Fixed ax = active_positions(g);
+    hipError_t e = hipMemcpyAsync(scratch, psi, sizeof(T) << n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    unsigned grid = grid_for(nfreeN << g.na, kBlock);
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(k_gate_big<R>, dim3(grid), dim3(kBlock), 0, s, psi, (const T *)scratch,
+                       nfreeN, g.na, fx, ax, (const T *)d_mat);
+  }
+  return hipGetLastError();
+}
+
+// ---- diagonal gates -------------------------------------------------------------
+
+// Multiply every amplitude whose control bits match by d[pattern of act bits].
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_diag(typename V2<R>::T *__restrict__ psi,
+                                                 uint64_t count, Fixed cx, Fixed ax,
+                                                 const typename V2<R>::T *__restrict__ d) {
+  using T = typename V2<R>::T;
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < count;
+       p += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t idx = expand(p, cx);
+    uint32_t pat = 0;
+    for (int j = 0; j < ax.nf; ++j) pat |= (uint32_t)((idx >> ax.pos[j]) & 1ull) << j;
+    const T dv = d[pat];
+    T a = psi[idx];
+    T r = czero<T>();
+    r = cmla(r, dv.x, dv.y, a);
+    psi[idx] = r;
+  }
+}
+
+template <typename R>
+hipError_t launch_gate_diag(void *psi, int n, const Gate &g, void *d_mat, hipStream_t s) {
+  using T = typename V2<R>::T;
+  const Fixed cx = make_fixed(g.cmask, g.cval);
+  const Fixed ax = active_positions(g);
+  const uint64_t count = 1ull << (n - cx.nf);
+  unsigned grid = grid_for(count, kBlock * 4);
+  if (grid > 262144) grid = 262144;
+  hipLaunchKernelGGL(k_diag<R>, dim3(grid), dim3(kBlock), 0, s, (T *)psi, count, cx, ax,
+                     (const T *)d_mat);
+  return hipGetLastError();
+}
+
+// Up to kMaxSparse (pattern, value) exceptions of a diagonal that is 1
+// elsewhere.
+constexpr int kMaxSparse = 16;
+template <typename R>
+struct SparseArg {
+  uint64_t pat[kMaxSparse];
+  R re[kMaxSparse], im[kMaxSparse];
+  int count;
+};
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_sparse(typename V2<R>::T *__restrict__ psi,
+                                                   uint64_t per, Fixed fx, SparseArg<R> sp) {
+  using T = typename V2<R>::T;
+  const uint64_t total = per * (uint64_t)sp.count;
+  for (uint64_t w = (uint64_t)blockIdx.x * kBlock + threadIdx.x; w < total;
+       w += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t e = w / per, p = w % per;
+    const uint64_t idx = expand(p, fx) | sp.pat[e];
+    T a = psi[idx];
+    T r = czero<T>();
+    r = cmla(r, sp.re[e], sp.im[e], a);
+    psi[idx] = r;
+  }
+}
+
+template <typename R>
+hipError_t launch_gate_sparse(void *psi, int n, const Gate &g, hipStream_t s) {
+  using T = typename V2<R>::T;
+  SparseArg<R> sp{};
+  sp.count = (int)g.spat.size();
+  if (sp.count > kMaxSparse) return hipErrorInvalidValue;
+  if (sp.count == 0) return hipSuccess;
+  for (int e = 0; e < sp.count; ++e) {
+    sp.pat[e] = g.spat[e];
+    sp.re[e] = (R)g.m[e].re;
+    sp.im[e] = (R)g.m[e].im;
+  }
+  const Fixed fx = make_fixed(g.smask, 0);
+  const uint64_t per = 1ull << (n - fx.nf);
+  unsigned grid = grid_for(per * sp.count, kBlock * 4);
+  if (grid > 262144) grid = 262144;
+  hipLaunchKernelGGL(k_sparse<R>, dim3(grid), dim3(kBlock), 0, s, (T *)psi, per, fx, sp);
+  return hipGetLastError();
+}
+
+// ---- reductions -------------------------------------------------------------------
+
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double (*sh)[kBlock]) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) sh[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] = sh[c][0];
+}
+
+// Second stage: one block sums nblocks partials of N channels (fixed order).
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ part, int nblocks,
+                                                   double *__restrict__ out) {
+  __shared__ double sh[N][kBlock];
+  double v[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock)
+#pragma unroll
+    for (int c = 0; c < N; ++c) v[c] += part[(size_t)b * N + c];
+  block_sum<N>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < N; ++c) out[c] = v[c];
+}
+
+static inline int reduce_grid(uint64_t len) {
+  uint64_t g = (len + kBlock - 1) / kBlock;
+  if (g > (uint64_t)kReduceBlocks) g = kReduceBlocks;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_prob_split(const typename V2<R>::T *__restrict__ psi,
+                                                       uint64_t len, int target,
+                                                       double *__restrict__ part) {
+  __shared__ double sh[2][kBlock];
+  double v[2] = {0.0, 0.0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const auto a = psi[i];
+    const double re = (double)a.x, im = (double)a.y;
+    const double p = re * re + im * im;
+    if ((i >> target) & 1ull)
+      v[1] += p;
+    else
+      v[0] += p;
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = v[0];
+    part[2 * blockIdx.x + 1] = v[1];
+  }
+}
+
+template <typename R>
+hipError_t launch_prob_split(const void *psi, int n, int target, double *d_part, double *d_out,
+                             hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  const int g = reduce_grid(len);
+  hipLaunchKernelGGL(k_prob_split<R>, dim3(g), dim3(kBlock), 0, s,
+                     (const typename V2<R>::T *)psi, len, target, d_part);
+  hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kBlock), 0, s, (const double *)d_part, g, d_out);
+  return hipGetLastError();
+}
+
+// The collapse of try_measure (src/register.rs:324-339): zero the amplitudes
+// whose target bit differs from the outcome; divide the rest, component by
+// component, by sqrt(total).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_collapse(typename V2<R>::T *__restrict__ psi,
+                                                     uint64_t len, int target, int keep,
+                                                     double sq) {
+  using T = typename V2<R>::T;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    T a = psi[i];
+    if ((int)((i >> target) & 1ull) != keep) {
+      a = czero<T>();
+    } else {
+      a.x = (R)((double)a.x / sq);
+      a.y = (R)((double)a.y / sq);
+    }
+    psi[i] = a;
+  }
+}
+
+template <typename R>
+hipError_t launch_collapse(void *psi, int n, int target, int keep, double total, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  unsigned grid = grid_for(len, kBlock * 4);
+  if (grid > 262144) grid = 262144;
+  hipLaunchKernelGGL(k_collapse<R>, dim3(grid), dim3(kBlock), 0, s, (typename V2<R>::T *)psi, len,
+                     target, keep, sqrt(total));
+  return hipGetLastError();
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_norm(const typename V2<R>::T *__restrict__ psi,
+                                                 uint64_t len, double *__restrict__ part) {
+  __shared__ double sh[1][kBlock];
+  double v[1] = {0.0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const auto a = psi[i];
+    v[0] += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+  }
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+template <typename R>
+hipError_t launch_norm(const void *psi, int n, double *d_part, double *d_out, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  const int g = reduce_grid(len);
+  hipLaunchKernelGGL(k_norm<R>, dim3(g), dim3(kBlock), 0, s, (const typename V2<R>::T *)psi, len,
+                     d_part);
+  hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kBlock), 0, s, (const double *)d_part, g, d_out);
+  return hipGetLastError();
+}
+
+template <typename Ra, typename Rb>
+__global__ __launch_bounds__(kBlock) void k_inner(const typename V2<Ra>::T *__restrict__ a,
+                                                  const typename V2<Rb>::T *__restrict__ b,
+                                                  uint64_t len, double *__restrict__ part) {
+  __shared__ double sh[2][kBlock];
+  double v[2] = {0.0, 0.0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const auto x = a[i];
+    const auto y = b[i];
+    const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+    v[0] += xr * yr + xi * yi;  // Re conj(x) y
+    v[1] += xr * yi - xi * yr;  // Im conj(x) y
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = v[0];
+    part[2 * blockIdx.x + 1] = v[1];
+  }
+}
+
+template <typename Ra, typename Rb>
+static void inner_launch(const void *a, const void *b, uint64_t len, int g, double *d_part,
+                         hipStream_t s) {
+  hipLaunchKernelGGL((k_inner<Ra, Rb>), dim3(g), dim3(kBlock), 0, s,
+                     (const typename V2<Ra>::T *)a, (const typename V2<Rb>::T *)b, len, d_part);
+}
+
+hipError_t launch_inner(const void *a, bool a_c64, const void *b, bool b_c64, int n,
+                        double *d_part, double *d_out, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  const int g = reduce_grid(len);
+  if (!a_c64 && !b_c64) inner_launch<double, double>(a, b, len, g, d_part, s);
+  if (!a_c64 && b_c64) inner_launch<double, float>(a, b, len, g, d_part, s);
+  if (a_c64 && !b_c64) inner_launch<float, double>(a, b, len, g, d_part, s);
+  if (a_c64 && b_c64) inner_launch<float, float>(a, b, len, g, d_part, s);
+  hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kBlock), 0, s, (const double *)d_part, g, d_out);
+  return hipGetLastError();
+}
+
+// Count of amplitudes where !(|a_i - b_i| < tol) -- NaN counts as unequal,
+// as `e.norm() < 1e-8` is false for NaN in PartialEq (src/register.rs:411).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_neq(const typename V2<R>::T *__restrict__ a,
+                                                const typename V2<R>::T *__restrict__ b,
+                                                uint64_t len, double tol,
+                                                double *__restrict__ part) {
+  __shared__ double sh[1][kBlock];
+  double v[1] = {0.0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const auto x = a[i];
+    const auto y = b[i];
+    const double dr = (double)x.x - (double)y.x, di = (double)x.y - (double)y.y;
+    const double d = hypot(dr, di);
+    if (!(d < tol)) v[0] += 1.0;
+  }
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+template <typename R>
+hipError_t launch_neq_count(const void *a, const void *b, int n, double tol, double *d_part,
+                            double *d_out, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  const int g = reduce_grid(len);
+  hipLaunchKernelGGL(k_neq<R>, dim3(g), dim3(kBlock), 0, s, (const typename V2<R>::T *)a,
+                     (const typename V2<R>::T *)b, len, tol, d_part);
+  hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kBlock), 0, s, (const double *)d_part, g, d_out);
+  return hipGetLastError();
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_probs(const typename V2<R>::T *__restrict__ psi,
+                                                  uint64_t len, double *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const auto a = psi[i];
+    out[i] = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+  }
+}
+
+template <typename R>
+hipError_t launch_probabilities(const void *psi, int n, double *d_out, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  unsigned grid = grid_for(len, kBlock * 4);
+  if (grid > 262144) grid = 262144;
+  hipLaunchKernelGGL(k_probs<R>, dim3(grid), dim3(kBlock), 0, s, (const typename V2<R>::T *)psi,
+                     len, d_out);
+  return hipGetLastError();
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_from_c128(typename V2<R>::T *__restrict__ psi,
+                                                      const double *__restrict__ src,
+                                                      uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * kBlock) {
+    typename V2<R>::T v;
+    v.x = (R)src[2 * i];
+    v.y = (R)src[2 * i + 1];
+    psi[i] = v;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_to_c128(double *__restrict__ dst,
+                                                    const typename V2<R>::T *__restrict__ psi,
+                                                    uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const auto v = psi[i];
+    dst[2 * i] = (double)v.x;
+    dst[2 * i + 1] = (double)v.y;
+  }
+}
+
+template <typename R>
+hipError_t launch_convert_from_c128(void *psi, const double *src, uint64_t count, hipStream_t s) {
+  unsigned grid = grid_for(count, kBlock * 4);
+  if (grid > 262144) grid = 262144;
+  hipLaunchKernelGGL(k_from_c128<R>, dim3(grid), dim3(kBlock), 0, s, (typename V2<R>::T *)psi, src,
+                     count);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_convert_to_c128(double *dst, const void *psi, uint64_t count, hipStream_t s) {
+  unsigned grid = grid_for(count, kBlock * 4);
+  if (grid > 262144) grid = 262144;
+  hipLaunchKernelGGL(k_to_c128<R>, dim3(grid), dim3(kBlock), 0, s, dst,
+                     (const typename V2<R>::T *)psi, count);
+  return hipGetLastError();
+}
+
+// Histogram of |psi|^2 over the bit pattern of m qubits (bin bit j <->
+// qubit pos_order[j]).  Block b owns a contiguous range of the free counter
+// (the index bits outside the m qubits); thread t owns bins t, t + 256, ...
+// and sums them over the block's range in a fixed order, so the result is
+// deterministic without atomics.
+struct PosList {
+  uint8_t p[64];
+};
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_set_hist(const typename V2<R>::T *__restrict__ psi,
+                                                     Fixed fx, PosList po, int m, uint64_t nfree,
+                                                     uint64_t per_block,
+                                                     double *__restrict__ part) {
+  const int H = 1 << m;
+  const uint64_t j0 = (uint64_t)blockIdx.x * per_block;
+  uint64_t j1 = j0 + per_block;
+  if (j1 > nfree) j1 = nfree;
+  for (int h = threadIdx.x; h < H; h += kBlock) {
+    uint64_t dep = 0;
+    for (int j = 0; j < m; ++j)
+      if ((h >> j) & 1) dep |= 1ull << po.p[j];
+    double acc = 0.0;
+    for (uint64_t j = j0; j < j1; ++j) {
+      const auto a = psi[expand(j, fx) | dep];
+      acc += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+    }
+    part[(size_t)blockIdx.x * H + h] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_hist_finish(const double *__restrict__ part,
+                                                        int blocks, int H,
+                                                        double *__restrict__ out) {
+  for (int h = blockIdx.x * kBlock + threadIdx.x; h < H; h += gridDim.x * kBlock) {
+    double v = 0.0;
+    for (int b = 0; b < blocks; ++b) v += part[(size_t)b * H + h];
+    out[h] = v;
+  }
+}
+
+template <typename R>
+hipError_t launch_set_histogram(const void *psi, int n, const Fixed &fx, const uint8_t *pos_order,
+                                int m, double *d_part, int blocks, double *d_hist,
+                                hipStream_t s) {
+  const int H = 1 << m;
+  const uint64_t nfree = 1ull << (n - m);
+  const uint64_t per_block = (nfree + blocks - 1) / blocks;
+  PosList po{};
+  for (int j = 0; j < m; ++j) po.p[j] = pos_order[j];
+  hipLaunchKernelGGL(k_set_hist<R>, dim3(blocks), dim3(kBlock), 0, s,
+                     (const typename V2<R>::T *)psi, fx, po, m, nfree, per_block, d_part);
+  hipLaunchKernelGGL(k_hist_finish, dim3((H + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     (const double *)d_part, blocks, H, d_hist);
+  return hipGetLastError();
+}
+
+// |psi|^2 of the amplitudes expand(j, fx) for j < count.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_gather_probs(const typename V2<R>::T *__restrict__ psi,
+                                                         Fixed fx, uint64_t count,
+                                                         double *__restrict__ out) {
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < count;
+       j += (uint64_t)gridDim.x * kBlock) {
+    const auto a = psi[expand(j, fx)];
+    out[j] = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+  }
+}
+
+template <typename R>
+hipError_t launch_gather_probs(const void *psi, const Fixed &fx, uint64_t count, double *d_out,
+                               hipStream_t s) {
+  unsigned grid = grid_for(count, kBlock);
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(k_gather_probs<R>, dim3(grid), dim3(kBlock), 0, s,
+                     (const typename V2<R>::T *)psi, fx, count, d_out);
+  return hipGetLastError();
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_collapse_to(typename V2<R>::T *__restrict__ psi,
+                                                        uint64_t len, uint64_t idx, double sq) {
+  using T = typename V2<R>::T;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    if (i == idx) {
+      T a = psi[i];
+      a.x = (R)((double)a.x / sq);
+      a.y = (R)((double)a.y / sq);
+      psi[i] = a;
+    } else {
+      psi[i] = czero<T>();
+    }
+  }
+}
+
+template <typename R>
+hipError_t launch_collapse_to(void *psi, int n, uint64_t idx, double total, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  unsigned grid = grid_for(len, kBlock * 4);
+  if (grid > 262144) grid = 262144;
+  hipLaunchKernelGGL(k_collapse_to<R>, dim3(grid), dim3(kBlock), 0, s, (typename V2<R>::T *)psi,
+                     len, idx, sqrt(total));
+  return hipGetLastError();
+}
+
+// One block per chunk; within a block a fixed tree order.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_chunk_sums(const typename V2<R>::T *__restrict__ psi,
+                                                       uint64_t len, uint64_t chunk,
+                                                       double *__restrict__ out) {
+  __shared__ double sh[1][kBlock];
+  double v[1] = {0.0};
+  const uint64_t start = (uint64_t)blockIdx.x * chunk;
+  uint64_t end = start + chunk;
+  if (end > len) end = len;
+  for (uint64_t i = start + threadIdx.x; i < end; i += kBlock) {
+    const auto a = psi[i];
+    v[0] += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+  }
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
+template <typename R>
+hipError_t launch_chunk_sums(const void *psi, int n, uint64_t chunk, double *d_out,
+                             hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  const uint64_t nchunks = (len + chunk - 1) / chunk;
+  hipLaunchKernelGGL(k_chunk_sums<R>, dim3((unsigned)nchunks), dim3(kBlock), 0, s,
+                     (const typename V2<R>::T *)psi, len, chunk, d_out);
+  return hipGetLastError();
+}
+
+#define QS_INSTANTIATE(R)                                                                        \
+  template hipError_t launch_init_basis<R>(void *, int, uint64_t, hipStream_t);                 \
+  template hipError_t launch_init_product<R>(void *, int, const double *, hipStream_t);         \
+  template hipError_t launch_gate_dense<R>(void *, int, const Gate &, void *, void *,            \
+                                           hipStream_t);                                         \
+  template hipError_t launch_gate_diag<R>(void *, int, const Gate &, void *, hipStream_t);      \
+  template hipError_t launch_gate_sparse<R>(void *, int, const Gate &, hipStream_t);            \
+  template hipError_t launch_prob_split<R>(const void *, int, int, double *, double *,          \
+                                           hipStream_t);                                         \
+  template hipError_t launch_collapse<R>(void *, int, int, int, double, hipStream_t);           \
+  template hipError_t launch_norm<R>(const void *, int, double *, double *, hipStream_t);       \
+  template hipError_t launch_neq_count<R>(const void *, const void *, int, double, double *,    \
+                                          double *, hipStream_t);                                \
+  template hipError_t launch_probabilities<R>(const void *, int, double *, hipStream_t);        \
+  template hipError_t launch_convert_from_c128<R>(void *, const double *, uint64_t,             \
+                                                  hipStream_t);                                  \
+  template hipError_t launch_convert_to_c128<R>(double *, const void *, uint64_t, hipStream_t); \
+  template hipError_t launch_set_histogram<R>(const void *, int, const Fixed &, const uint8_t *, \
+                                              int, double *, int, double *, hipStream_t);        \
+  template hipError_t launch_gather_probs<R>(const void *, const Fixed &, uint64_t, double *,    \
+                                             hipStream_t);                                       \
+  template hipError_t launch_collapse_to<R>(void *, int, uint64_t, double, hipStream_t);        \
+  template hipError_t launch_chunk_sums<R>(const void *, int, uint64_t, double *, hipStream_t);
+
+QS_INSTANTIATE(double)
+QS_INSTANTIATE(float)
+
+}  // namespace qs

@@ -1,0 +1,55 @@
+// qs_sweep.h -- the fused sweep engine.
+//
+// A *sweep* is one pass over the state: each workgroup owns one tile of 2^T
+// amplitudes (the index bits at T chosen qubits, the tile qubits, vary inside
+// a tile; the others are fixed per workgroup), loads it once from HBM, applies
+// a run of gates to it, and stores it once.  Inside the tile every thread
+// keeps 2^A amplitudes in registers (A = T - 8 for 256 threads).  A *phase*
+// names the A tile qubits that are register bits; a dense gate on those
+// qubits is applied in registers with no communication, and moving to the
+// next phase is one exchange through LDS.  Controls and diagonal gates need
+// no data movement at all: every amplitude's global index is known, so they
+// are predicates and multiplies on whichever phase is current.
+//
+// The planner packs the pending gate list into sweeps, moving a gate ahead of
+// earlier ones only when it commutes with every gate it passes (on every
+// shared qubit both act diagonally), so the result equals applying the list
+// in order.  Gates the sweep kernel does not handle (dense on 3+ active
+// qubits, large diagonal tables) run as single-gate kernels between sweeps.
+#pragma once
+
+#include <functional>
+#include <vector>
+
+#include "qs_internal.h"
+
+namespace qs {
+
+struct SweepEngine;
+SweepEngine *sweep_engine_new();
+void sweep_engine_free(SweepEngine *e);
+
+struct SweepRunner {
+  int n = 0;
+  bool dtype_c64 = false;
+  void *psi = nullptr;
+  hipStream_t stream = nullptr;
+  // Copy host bytes to a device buffer that stays valid until the stream
+  // reaches the next launch.
+  std::function<hipError_t(const void *, size_t, void **)> upload;
+  // Launch one gate with the per-gate kernels.
+  std::function<hipError_t(const Gate &)> single;
+  // Called with true before and false after each sweep kernel launch
+  // (timing hooks); may be empty.
+  std::function<void(bool)> mark;
+};
+
+// Apply `gates` in order (with commuting reorders) through sweeps and
+// per-gate kernels.  Counters are incremented by the launches made.
+hipError_t sweep_execute(SweepEngine *e, SweepRunner &run, const std::vector<Gate> &gates,
+                         uint64_t *sweeps, uint64_t *kernels);
+
+// Tile width used for a dtype (log2 amplitudes per workgroup).
+int sweep_tile_bits(bool c64);
+
+}  // namespace qs

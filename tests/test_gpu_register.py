@@ -1,0 +1,409 @@
+"""GPU parity: the HIP register against the CPU oracle.
+
+Bar: complex128 results within 1e-12 per amplitude of the oracle (the
+north-star tolerance for the drop-in); complex64 within its own rounding
+(infidelity against the FP64 oracle <= 1e-6).  Indices and outcomes are
+compared exactly.  Every case runs with gate fusion on (the default sweep
+engine) and off (one kernel per gate).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from quaru_amd import OperationError, QuaruPanic, Register, circuits
+from quaru_amd import operation as ops
+from tests.conftest import random_state, random_unitary
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def oracle_run(state, opl):
+    for op in opl:
+        state = oracle.apply_fast(state, op.matrix(), op.targets())
+    return state
+
+
+def maxdiff(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
+
+
+def infid(a, b):
+    ov = np.vdot(a, b)
+    return 1.0 - abs(ov) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+# ---- construction --------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_new_from_int_from_qubits(dtype):
+    bits = [True, False, True, True, False, False, True]
+    assert maxdiff(Register.new(bits, dtype).state, oracle.new(bits)) == 0
+    assert maxdiff(Register.from_int(6, 37, dtype).state, oracle.from_int(6, 37)) == 0
+    f = 1 / math.sqrt(2)
+    qs = [[1, 0], [0, 1], [f, f], [f, -1j * f]]
+    r = Register.from_qubits(qs, dtype)
+    _, ref, _ = oracle.try_from_qubits(qs)
+    assert maxdiff(r.state, ref) < (TOL if dtype == "c128" else 1e-7)
+    with pytest.raises(OperationError) as e:
+        Register.try_from_qubits([[1, 0], [1, 1]], dtype)
+    assert e.value == OperationError.InvalidQubit(1)
+    with pytest.raises(QuaruPanic):
+        Register.from_qubits([[1, 0, 0]], dtype)
+
+
+def test_zero_qubit_register():
+    r = Register.new([])
+    assert r.size() == 0 and maxdiff(r.state, [1]) == 0
+    r.apply(ops.Operation.new(np.array([[1j]]), []))
+    assert maxdiff(r.state, [1j]) == 0
+    with pytest.raises(OperationError):
+        r.try_measure(0)
+
+
+# ---- validation (src/register.rs:141-153) ------------------------------------------
+
+def test_validation_precedence_matches_oracle():
+    r = Register.new([False] * 4)
+    cases = [
+        (np.eye(16), [9, 3, 1, 3]),
+        (np.eye(16), [3, 1, 3, 1]),
+        (np.eye(8), [1, 7, 5]),
+        (np.eye(4), [2, 2]),
+    ]
+    for m, t in cases:
+        rc, _, a, _ = oracle.try_apply_fast(oracle.new([False] * 4), m, t)
+        with pytest.raises(OperationError) as e:
+            r.try_apply_fast(ops.Operation(m, t))
+        assert e.value == OperationError.InvalidTarget(a), (t, e.value)
+    with pytest.raises(OperationError) as e:
+        r.try_apply_fast(ops.Operation(np.eye(2), [0, 1]))
+    assert e.value == OperationError.InvalidDimensions(2, 2)
+    with pytest.raises(QuaruPanic):
+        r.apply(ops.cnot(1, 4))
+    with pytest.raises(OperationError) as e:
+        r.try_apply_all(ops.cnot(0, 1))
+    assert e.value == OperationError.InvalidArity(2)
+    # src/register.rs:437-445
+    r2 = Register.new([False, False])
+    with pytest.raises(OperationError) as e:
+        r2.try_apply(ops.cnot(1, 2))
+    assert e.value == OperationError.InvalidTarget(2)
+    r2.try_apply(ops.cnot(0, 1))
+
+
+# ---- gate parity at small n, every gate class ------------------------------------
+
+def gate_zoo(rng, n):
+    """Gates exercising each classified kind and kernel."""
+    zoo = []
+    for q in rng.permutation(n)[:4]:
+        zoo.append(ops.Operation(random_unitary(rng, 2), [int(q)]))
+    t = [int(x) for x in rng.permutation(n)[:2]]
+    zoo.append(ops.Operation(random_unitary(rng, 4), t))
+    for k in (3, 4, 5, 6, 7):
+        t = [int(x) for x in rng.permutation(n)[:k]]
+        zoo.append(ops.Operation(random_unitary(rng, 1 << k), t))
+    a, b, c, d = [int(x) for x in rng.permutation(n)[:4]]
+    zoo += [ops.cnot(a, b), ops.cnx([a, b], c), ops.cnz([a, b], c), ops.cnz([a, b, c], d),
+            ops.swap(a, c), ops.phase(b), ops.pauli_y(d), ops.pauli_z(a), ops.identity(c),
+            ops.rz(0.3, b), ops.to_controlled(ops.Operation(random_unitary(rng, 4), [a, b]), c),
+            ops.hadamard_transform([a, c, d])]
+    diag = np.exp(1j * rng.uniform(0, 2 * np.pi, 8))
+    zoo.append(ops.Operation(np.diag(diag), [b, d, a]))
+    sparse = np.ones(16, dtype=np.complex128)
+    sparse[5] = -1
+    sparse[9] = 1j
+    zoo.append(ops.Operation(np.diag(sparse), [c, a, d, b]))
+    zoo.append(ops.Operation.new(np.array([[np.exp(0.4j)]]), []))
+    return zoo
+
+
+@pytest.mark.parametrize("fusion", [True, False])
+@pytest.mark.parametrize("n", [8, 12, 13])
+def test_gate_zoo_parity_c128(n, fusion):
+    rng = np.random.default_rng(100 + n)
+    psi = random_state(rng, n)
+    r = Register.new([False] * n).set_fusion(fusion)
+    r.state = psi
+    zoo = gate_zoo(rng, n)
+    ref = psi
+    for op in zoo:
+        r.apply(op)
+        ref = oracle.apply_fast(ref, op.matrix(), op.targets())
+    assert maxdiff(r.state, ref) < TOL
+
+
+@pytest.mark.parametrize("n", [12, 14])
+def test_gate_zoo_parity_c64(n):
+    rng = np.random.default_rng(200 + n)
+    psi = random_state(rng, n)
+    r = Register.new([False] * n, "c64")
+    r.state = psi
+    ref = psi
+    for op in gate_zoo(rng, n):
+        r.apply(op)
+        ref = oracle.apply_fast(ref, op.matrix(), op.targets())
+    got = r.state
+    assert infid(got, ref) < 1e-6
+    assert maxdiff(got, ref) < 1e-5
+
+
+# ---- the sweep engine on random circuits (configs 1 and 3 structure) -------------
+
+@pytest.mark.parametrize("n,layers,triples", [(12, 4, 0), (14, 3, 4), (16, 2, 8), (18, 2, 8)])
+def test_random_circuit_sweeps(n, layers, triples):
+    w = circuits.random_circuit(n, layers, seed=n * 31 + layers, triples=triples, measure=False)
+    ref = oracle_run(oracle.new([False] * n), w.ops)
+    for fusion in (True, False):
+        r = Register.new([False] * n).set_fusion(fusion)
+        for op in w.ops:
+            r.apply(op)
+        st = r.stats()
+        assert maxdiff(r.state, ref) < TOL, fusion
+        if fusion:
+            assert st["sweeps"] > 0
+            assert st["gates"] == len(w.ops)
+
+
+def test_golden_config1_on_gpu(golden):
+    g = golden("c1_12q.npz")
+    w = circuits.random_circuit(12, 20, seed=int(g["seed"]))
+    for fusion in (True, False):
+        r = Register.new([False] * 12).set_fusion(fusion)
+        for op in w.ops:
+            r.apply(op)
+        assert maxdiff(r.state, g["amplitudes"]) < TOL
+        idx = r.measure_sequence(range(12), w.uniforms)
+        assert idx == int(g["index"])
+        assert maxdiff(r.state, g["collapsed"]) < TOL
+
+
+def test_golden_config1_c64(golden):
+    g = golden("c1_12q.npz")
+    w = circuits.random_circuit(12, 20, seed=int(g["seed"]))
+    r = Register.new([False] * 12, "c64")
+    for op in w.ops:
+        r.apply(op)
+    assert infid(r.state, g["amplitudes"]) < 1e-6
+
+
+def test_golden_config3_mini(golden):
+    g = golden("c3_mini.npz")
+    w = circuits.random_circuit(10, 3, seed=7, triples=4, measure=False)
+    r = Register.new([False] * 10)
+    for op in w.ops:
+        r.apply(op)
+    assert maxdiff(r.state, g["amplitudes"]) < TOL
+
+
+# ---- apply_all -------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [2, 6, 13])
+def test_apply_all(n):
+    r = Register.new([False] * n)
+    r.apply_all(ops.hadamard(0))
+    assert maxdiff(r.state, np.full(1 << n, 2 ** (-n / 2))) < TOL
+    rng = np.random.default_rng(n)
+    u = random_unitary(rng, 2)
+    psi = random_state(rng, n)
+    r.state = psi
+    r.apply_all(ops.Operation(u, [0]))
+    ref = psi
+    for q in range(n):
+        ref = oracle.apply_fast(ref, u, [q])
+    assert maxdiff(r.state, ref) < TOL
+
+
+# ---- measurement -------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_measure_matches_oracle(dtype):
+    rng = np.random.default_rng(11)
+    n = 13
+    psi = random_state(rng, n)
+    for target in (0, 5, 12):
+        for x in (0.1, 0.5, 0.93):
+            r = Register.new([False] * n, dtype)
+            r.state = psi
+            res = r.measure(target, x)
+            _, rr, ref, _ = oracle.try_measure(psi, target, x)
+            assert res == rr
+            tol = TOL if dtype == "c128" else 1e-6
+            assert maxdiff(r.state, ref) < tol
+
+
+def test_measure_basis_and_bell():
+    rng = np.random.default_rng(3)
+    for n in (4, 8):
+        bits = [bool(b) for b in rng.integers(0, 2, n)]
+        r = Register.new(bits, seed=1)
+        assert [r.measure(q) for q in range(n)] == bits
+    for i in range(6):
+        j = (i + 2) % 6
+        r = Register.new([False] * 6, seed=i)
+        r.apply(ops.hadamard(i)).apply(ops.cnot(i, j))
+        assert r.measure(i) == r.measure(j)
+
+
+@pytest.mark.parametrize("n", [10, 14, 20])
+def test_measure_sequence_matches_oracle(n):
+    rng = np.random.default_rng(n)
+    psi = random_state(rng, n)
+    for order in (list(range(n)), list(range(n))[::-1], [int(x) for x in rng.permutation(n)[:7]]):
+        u = rng.random(len(order))
+        r = Register.new([False] * n)
+        r.state = psi
+        idx = r.measure_sequence(order, u)
+        ridx, ref = oracle.measure_sequence(psi, order, u)
+        assert idx == ridx
+        assert maxdiff(r.state, ref) < TOL
+
+
+def test_measure_sequence_invalid_target_after_prefix():
+    r = Register.new([True, False, True])
+    with pytest.raises(OperationError) as e:
+        r.measure_sequence([0, 1, 5, 2], [0.5] * 4)
+    assert e.value == OperationError.InvalidTarget(5)
+    assert maxdiff(r.state, oracle.new([True, False, True])) == 0
+
+
+def test_sample_cdf():
+    rng = np.random.default_rng(8)
+    n = 18
+    psi = random_state(rng, n)
+    r = Register.new([False] * n)
+    r.state = psi
+    u = rng.random(64)
+    got = r.sample(u)
+    p = np.abs(psi) ** 2
+    cdf = np.cumsum(p)
+    exp = np.searchsorted(cdf, u * cdf[-1], side="right")
+    assert np.array_equal(got, exp)
+
+
+# ---- reductions and equality ---------------------------------------------------------
+
+def test_norm_inner_equal_probabilities():
+    rng = np.random.default_rng(21)
+    n = 15
+    a, b = random_state(rng, n), random_state(rng, n)
+    ra, rb = Register.new([False] * n), Register.new([False] * n)
+    ra.state, rb.state = a, b
+    assert abs(ra.norm_sqr() - 1.0) < 1e-13
+    assert abs(ra.inner(rb) - np.vdot(a, b)) < 1e-13
+    assert not (ra == rb)
+    rc = ra.clone()
+    assert ra == rc
+    assert maxdiff(ra.probabilities(), np.abs(a) ** 2) < 1e-15
+    ra64 = Register.new([False] * n, "c64")
+    ra64.state = a
+    assert abs(ra64.inner(ra) - 1.0) < 1e-6
+    assert ra64.infidelity(ra) < 1e-10
+
+
+# ---- reference property tests re-expressed on the GPU -------------------------------
+
+def test_reference_properties():
+    # tests/register_operation_tests.rs:44-50, 99-107, 143-181, 184-208
+    for i in range(3):
+        r = Register.new([False] * 3, seed=i)
+        r.apply(ops.not_(i))
+        assert r.measure(i)
+        r = Register.new([False] * 3)
+        expected = r.clone()
+        h = ops.hadamard(i)
+        assert r.apply(h).apply(h) == expected
+    for op in (ops.hadamard(2), ops.not_(4), ops.identity(1), ops.cnot(1, 3), ops.swap(0, 5)):
+        r = Register.new([False] * 6)
+        expected = r.clone()
+        r.apply(op).apply(op)
+        assert r == expected
+    for i in range(5):
+        for j in range(5):
+            if i == j:
+                continue
+            q = [False] * 6
+            q[i] = True
+            r = Register.new(q, seed=7)
+            r.apply(ops.swap(i, j))
+            assert [r.measure(k) for k in range(5)] == [k == j for k in range(5)]
+    rng = np.random.default_rng(0)
+    for n in range(2, 7):
+        init = [bool(b) for b in rng.integers(0, 2, 6)]
+        r = Register.new(init, seed=n)
+        r.apply(ops.cnx(list(range(n - 1)), n - 1))
+        ctl = all(r.measure(i) for i in range(n - 1))
+        t = r.measure(n - 1)
+        assert (t != init[n - 1]) if ctl else (t == init[n - 1])
+
+
+def test_apply_circuit_teleportation():
+    # examples/quantum_teleportation.rs:66-119 through apply_circuit
+    rng = np.random.default_rng(4)
+    for trial in range(6):
+        th, ph = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi)
+        q0 = np.array([math.cos(th / 2), math.sin(th / 2) * np.exp(1j * ph)])
+        r = Register.new([False] * 3, seed=trial)
+        r.state = np.kron(np.kron([1, 0], [1, 0]), q0)
+        c = ops.QuantumCircuit()
+        for op in (ops.hadamard(2), ops.cnot(2, 1), ops.cnot(0, 1), ops.hadamard(0)):
+            c.add_operation(op)
+        c.add_measurement(1)
+        c.add_measurement(0)
+        c.add_conditional_operation(ops.not_(2), [(1, True)])
+        c.add_conditional_operation(ops.pauli_z(2), [(0, True)])
+        r.apply_circuit(c)
+        s = r.state
+        idx = np.arange(8)
+        beta = s[(idx >> 2) & 1 == 1].sum()
+        alpha = s[(idx >> 2) & 1 == 0].sum()
+        p = abs(alpha) ** 2 + abs(beta) ** 2
+        assert abs(alpha / math.sqrt(p) - q0[0]) < 1e-8
+        assert abs(beta / math.sqrt(p) - q0[1]) < 1e-8
+
+
+def test_grover_reference_ops_closed_form():
+    # examples/grovers.rs with its dense k = n oracle and diffusion matrices
+    n, t, w = 8, 5, 77
+    N = 1 << n
+    orc = np.eye(N)
+    orc[w, w] = -1
+    dif = -np.eye(N)
+    dif[0, 0] = 1
+    r = Register.new([False] * n)
+    r.apply_all(ops.hadamard(0))
+    for _ in range(t):
+        r.apply(ops.Operation(orc, list(range(n))))
+        r.apply_all(ops.hadamard(0))
+        r.apply(ops.Operation(dif, list(range(n))))
+        r.apply_all(ops.hadamard(0))
+    aw, ao = circuits.grover_closed_form(n, t, w)
+    s = r.state
+    assert abs(s[w] - aw) < TOL and maxdiff(np.delete(s, w), ao) < TOL
+
+
+def test_shor_slow_period_finding_register_ops():
+    # examples/shors.rs:180-234 for N = 15, a = 7: 12 qubits, an 8-qubit
+    # dense Hadamard transform, 8 controlled 5-qubit modmul gates and a dense
+    # 256 x 256 QFT, compared with the oracle.
+    number, a, nb = 15, 7, 4
+    r = Register.new([False] * (3 * nb))
+    opl = [ops.hadamard_transform(list(range(2 * nb))), ops.not_(2 * nb)]
+    targets = list(range(2 * nb, 3 * nb))
+    for i in range(2 * nb):
+        apm = pow(a, 1 << i, number)
+        g = ops.to_quantum_gate(lambda x, apm=apm: (x * apm) % number if x < number else x,
+                                targets)
+        opl.append(ops.to_controlled(g, i))
+    m = 1 << (2 * nb)
+    wq = np.exp(2j * math.pi / m)
+    j = np.arange(m)
+    opl.append(ops.Operation(wq ** np.outer(j, j) / math.sqrt(m), list(range(2 * nb))))
+    for op in opl:
+        r.apply(op)
+    ref = oracle_run(oracle.new([False] * 12), opl)
+    assert maxdiff(r.state, ref) < 1e-11
