@@ -1181,6 +1181,8 @@ int qs_set_fusion(qs_register *r, int enabled) {
 int qs_stats(qs_register *r, uint64_t *sweeps, uint64_t *gate_kernels, uint64_t *gates,
              int reset) {
   if (!r) return fail(QS_ERR_ARGUMENT, 0, 0, "register is NULL");
+  int rc = flush(r);
+  if (rc) return rc;
   if (sweeps) *sweeps = r->st_sweeps;
   if (gate_kernels) *gate_kernels = r->st_kernels;
   if (gates) *gates = r->st_gates;
