@@ -49,7 +49,7 @@ struct SweepRunner {
 hipError_t sweep_execute(SweepEngine *e, SweepRunner &run, const std::vector<Gate> &gates,
                          uint64_t *sweeps, uint64_t *kernels);
 
-// Tile width used for a dtype (log2 amplitudes per workgroup).
-int sweep_tile_bits(bool c64);
+// Register bits per lane used for a dtype (a tile is 2^(6 + bits) amplitudes).
+int sweep_reg_bits(bool c64);
 
 }  // namespace qs

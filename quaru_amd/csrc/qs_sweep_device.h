@@ -1,51 +1,80 @@
 // qs_sweep_device.h -- device side of the fused sweep engine: the sweep
 // descriptor layout and the sweep kernel.  Included by qs_sweep.hip only.
 //
-// Amplitudes live in per-thread register arrays re[]/im[] of plain scalars
-// (complex structs in registers would be spilled to scratch by the compiler
-// whenever they are selected as a whole).
+// A sweep is one pass over the state.  Each wave (64 lanes) owns a tile of
+// 2^(6+A) amplitudes: every lane keeps 2^A of them in registers.  The tile
+// index bits are assigned as
+//   lane bits 0..3  <- qubits 0..3 (16 lanes read 16 consecutive amplitudes:
+//                      coalesced 256-byte (c128) / 128-byte (c64) runs),
+//   lane bits 4..5  <- two more tile qubits,
+//   register slots  <- A more tile qubits,
+// and the remaining index bits come from the tile id.  A gate on a register
+// qubit pairs registers inside a lane; a gate on a lane qubit fetches the
+// partner lane's amplitude with ds_bpermute (one code path for every lane
+// qubit) and each lane computes its own output.  There is no LDS tile and no
+// barrier: a sweep is load, ops, store.
+//
+// Controls are split into the part on register slots, which is the same for
+// the whole wave (scalar arithmetic), and the part on every other index bit,
+// which is one predicate per lane and op.
+//
+// The descriptor is read through the constant address space with 32/64-bit
+// fields only, so the compiler keeps every descriptor read on scalar loads.
+// Amplitudes live in per-lane register arrays re[]/im[] of plain scalars
+// (complex structs selected as a whole would be spilled to scratch).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "qs_internal.h"
 
+#define QS_CONST __attribute__((address_space(4)))
+
 namespace qs {
 namespace sweep {
 
-constexpr int kThreads = 256;
-constexpr int kMaxPhases = 16;
+constexpr int kThreads = 256;  // 4 waves = 4 independent tiles per workgroup
+constexpr int kWave = 64;
+constexpr int kLaneBits = 6;
+constexpr int kMaxA = 5;
 
 enum : uint32_t {
-  OP_DENSE1 = 1,  // 2x2 on one register slot
-  OP_DENSE2 = 2,  // 4x4 on two register slots (matrix bit 0 <-> s0)
-  OP_X1 = 3,      // exchange on one register slot (Pauli X / CNOT / Toffoli)
-  OP_SPARSE = 4,  // multiply by m[0] where (idx & cmask) == cval
-  OP_DIAG = 5     // multiply by m[pattern of act bits] where (idx & cmask) == cval
+  OP_U1 = 1,      // uncontrolled 2x2 on one qubit (register slot or lane bit)
+  OP_C1 = 2,      // controlled 2x2 on one qubit
+  OP_X1 = 3,      // (controlled) exchange: Pauli X / CNOT / Toffoli targets
+  OP_DENSE2 = 4,  // 4x4 on two register slots (matrix bit 0 <-> s0)
+  OP_SPARSE = 5,  // multiply by pool value where (idx & cmask) == cval
+  OP_DIAG = 6     // multiply by pool[pattern of 1-2 act bits] where (idx & cmask) == cval
 };
 
-template <typename R>
-struct SweepOp {
-  uint32_t type;
-  uint8_t s0, s1, na, pad0;
-  uint8_t act[4];
-  uint32_t pad1;
+// Slot encoding in s0/s1: 0..A-1 register slot, kLaneSlot + k lane bit k.
+constexpr uint32_t kLaneSlot = 8;
+
+// 64-byte op record; complex values live in a pool of R scalars
+// ({re, im} interleaved, row-major) at pool[moff].
+struct OpRec {
+  uint32_t type, s0, s1, na;
+  uint32_t act[4];
+  uint32_t moff, pad0;
   uint64_t cmask, cval;
-  R m[32];  // row-major complex matrix, {re, im} interleaved
-};
-
-struct PhaseDesc {
-  uint8_t reg[8];  // local bit of register slot j
-  uint16_t op_begin, op_end;
-  uint32_t pad;
+  uint32_t pad1[2];
 };
 
 struct SweepHeader {
-  Fixed tile;        // tile qubits, sorted; expanding blockIdx gives the tile base
-  uint8_t tpos[16];  // local bit -> global qubit
-  int nphase;
-  PhaseDesc ph[kMaxPhases];
+  uint32_t tpos[16];  // the tile qubits, ascending (tile-id bits fill the rest)
+  uint32_t nops, npool;
+  uint64_t ntiles;
+  uint64_t gb[8];     // global bit of register slot j
+  uint64_t lg[8];     // global bit of lane bit i (i < 6)
 };
+
+template <typename R>
+__host__ __device__ inline size_t blob_bytes(int nops, int npool) {
+  const size_t h = (sizeof(SweepHeader) + 15) & ~(size_t)15;
+  const size_t o = (size_t)nops * sizeof(OpRec);
+  const size_t p = ((size_t)npool * sizeof(R) + 15) & ~(size_t)15;
+  return h + o + p;
+}
 
 template <typename R>
 struct SV2;
@@ -58,72 +87,71 @@ struct SV2<float> {
   using T = float2;
 };
 
-__device__ __forceinline__ uint64_t sw_expand(uint64_t p, const Fixed &f) {
-  for (int i = 0; i < f.nf; ++i) {
-    const uint64_t lo = p & ((1ull << f.pos[i]) - 1ull);
+// Base index of a tile: insert a zero at each of the T tile positions.
+template <int T>
+__device__ __forceinline__ uint64_t tile_base(uint64_t p, const QS_CONST SweepHeader &h) {
+#pragma unroll
+  for (int i = 0; i < T; ++i) {
+    const uint64_t lo = p & ((1ull << h.tpos[i]) - 1ull);
     p = ((p ^ lo) << 1) | lo;
   }
-  return p | f.set;
+  return p;
 }
 
-// Register slot <-> local/global bit offsets for the current phase.
 template <int A>
-struct Map {
-  uint32_t lthr;   // local index bits owned by this thread
-  uint64_t gthr;   // the same bits as global index bits
-  uint32_t lb[A];  // local bit of slot j
-  uint64_t gb[A];  // global bit of slot j
+struct Regs {
+  uint64_t gb[A];  // global bit of register slot j (wave-uniform)
+  uint64_t mask;   // OR of gb[]
 };
 
-template <int T, int A>
-__device__ __forceinline__ void make_map(const SweepHeader &h, int p, int t, Map<A> &mp) {
-  uint32_t regmask = 0;
-#pragma unroll
-  for (int j = 0; j < A; ++j) {
-    const int b = h.ph[p].reg[j];
-    regmask |= 1u << b;
-    mp.lb[j] = 1u << b;
-    mp.gb[j] = 1ull << h.tpos[b];
-  }
-  uint32_t l = 0;
-  int i = 0;
-#pragma unroll
-  for (int b = 0; b < T; ++b) {
-    if (!((regmask >> b) & 1u)) {
-      if ((t >> i) & 1) l |= 1u << b;
-      ++i;
-    }
-  }
-  uint64_t g = 0;
-#pragma unroll
-  for (int b = 0; b < T; ++b)
-    if ((l >> b) & 1u) g |= 1ull << h.tpos[b];
-  mp.lthr = l;
-  mp.gthr = g;
-}
-
 template <int A>
-__device__ __forceinline__ uint64_t goff(const Map<A> &mp, int r) {
+__device__ __forceinline__ uint64_t goff(const Regs<A> &g, int r) {
   uint64_t o = 0;
 #pragma unroll
   for (int j = 0; j < A; ++j)
-    if ((r >> j) & 1) o |= mp.gb[j];
+    if ((r >> j) & 1) o |= g.gb[j];
   return o;
+}
+
+struct Ctx {
+  uint64_t gbase;  // tile base | this lane's lane-bit offsets (no register bits)
+  int lane;
+};
+
+// on(r) = lane_ok && ((goff(r) & cmr) == cvr): the second factor is uniform.
+struct Pred {
+  bool lane_ok;
+  uint64_t cmr, cvr;
+};
+
+template <int A>
+__device__ __forceinline__ Pred make_pred(const QS_CONST OpRec &op, const Ctx &c,
+                                          const Regs<A> &g) {
+  const uint64_t cm = op.cmask, cv = op.cval;
+  const uint64_t cmo = cm & ~g.mask;
+  Pred p;
+  p.cmr = cm & g.mask;
+  p.cvr = cv & g.mask;
+  p.lane_ok = (c.gbase & cmo) == (cv & cmo);
+  return p;
 }
 
 template <int A>
-__device__ __forceinline__ uint32_t loff(const Map<A> &mp, int r) {
-  uint32_t o = 0;
-#pragma unroll
-  for (int j = 0; j < A; ++j)
-    if ((r >> j) & 1) o |= mp.lb[j];
-  return o;
+__device__ __forceinline__ bool pred_on(const Pred &p, const Regs<A> &g, int r) {
+  return p.lane_ok && ((goff<A>(g, r) & p.cmr) == p.cvr);
 }
 
-// LDS slot of local index l: the low 4 bits are XOR-ed with bits 4..7, so
-// both lanes-along-the-low-bits and lanes-along-bits-4..7 layouts are free
-// of bank conflicts.
-__device__ __forceinline__ uint32_t swz(uint32_t l) { return l ^ ((l >> 4) & 15u); }
+// Partner lane's value: `addr` = 4 * (lane ^ (1 << k)).
+__device__ __forceinline__ double xchg(double v, int addr) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float xchg(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
 
 template <typename R, int NA>
 struct Amp {
@@ -140,12 +168,14 @@ __device__ __forceinline__ void cfma(R &ore, R &oim, R mr, R mi, R xr, R xi) {
   oim = fma(mi, xr, oim);
 }
 
-template <int J, int A, typename R>
-__device__ __forceinline__ void op_dense1(Amp<R, 1 << A> &a, const SweepOp<R> &op, uint64_t gbase,
-                                          const Map<A> &mp) {
-  const R m0r = op.m[0], m0i = op.m[1], m1r = op.m[2], m1i = op.m[3];
-  const R m2r = op.m[4], m2i = op.m[5], m3r = op.m[6], m3i = op.m[7];
-  const uint64_t cm = op.cmask, cv = op.cval;
+// 2x2 on register slot J; kCtl selects the controlled form.
+template <int J, bool kCtl, int A, typename R>
+__device__ __forceinline__ void op_1q_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                          const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+  const R m0r = m[0], m0i = m[1], m1r = m[2], m1i = m[3];
+  const R m2r = m[4], m2i = m[5], m3r = m[6], m3i = m[7];
+  Pred p{};
+  if constexpr (kCtl) p = make_pred<A>(op, c, g);
   constexpr int bj = 1 << J;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
@@ -156,24 +186,64 @@ __device__ __forceinline__ void op_dense1(Amp<R, 1 << A> &a, const SweepOp<R> &o
     cfma(ur, ui, m1r, m1i, yr, yi);
     cfma(wr, wi, m2r, m2i, xr, xi);
     cfma(wr, wi, m3r, m3i, yr, yi);
-    const bool on = cm == 0 || (((gbase | goff<A>(mp, r)) & cm) == cv);
-    a.re[r] = on ? ur : xr;
-    a.im[r] = on ? ui : xi;
-    a.re[r | bj] = on ? wr : yr;
-    a.im[r | bj] = on ? wi : yi;
+    if constexpr (kCtl) {
+      const bool on = pred_on<A>(p, g, r);
+      a.re[r] = on ? ur : xr;
+      a.im[r] = on ? ui : xi;
+      a.re[r | bj] = on ? wr : yr;
+      a.im[r | bj] = on ? wi : yi;
+    } else {
+      a.re[r] = ur;
+      a.im[r] = ui;
+      a.re[r | bj] = wr;
+      a.im[r | bj] = wi;
+    }
+  }
+}
+
+// 2x2 on a lane qubit: a lane with the bit clear holds x and gets the
+// partner's y: out = m00 x + m01 y; a lane with the bit set holds y and
+// computes out = m10 x + m11 y from the partner's x.
+template <bool kCtl, int A, typename R>
+__device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                           const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+  const int k = (int)op.s0 - (int)kLaneSlot;
+  const bool hi = (c.lane >> k) & 1;
+  const int addr = (c.lane ^ (1 << k)) << 2;
+  const R sr = hi ? m[6] : m[0], si = hi ? m[7] : m[1];
+  const R pr = hi ? m[4] : m[2], pi = hi ? m[5] : m[3];
+  Pred p{};
+  if constexpr (kCtl) p = make_pred<A>(op, c, g);
+#pragma unroll
+  for (int r = 0; r < (1 << A); ++r) {
+    const R xr = a.re[r], xi = a.im[r];
+    const R yr = xchg(xr, addr), yi = xchg(xi, addr);
+    R ur = 0, ui = 0;
+    cfma(ur, ui, sr, si, xr, xi);
+    cfma(ur, ui, pr, pi, yr, yi);
+    if constexpr (kCtl) {
+      const bool on = pred_on<A>(p, g, r);
+      a.re[r] = on ? ur : xr;
+      a.im[r] = on ? ui : xi;
+    } else {
+      a.re[r] = ur;
+      a.im[r] = ui;
+    }
+    // keep one register's exchange live at a time
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
 template <int J, int A, typename R>
-__device__ __forceinline__ void op_x1(Amp<R, 1 << A> &a, const SweepOp<R> &op, uint64_t gbase,
-                                      const Map<A> &mp) {
-  const uint64_t cm = op.cmask, cv = op.cval;
+__device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                          const Ctx &c, const Regs<A> &g) {
+  const Pred p = make_pred<A>(op, c, g);
   constexpr int bj = 1 << J;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     if ((r >> J) & 1) continue;
     const R xr = a.re[r], xi = a.im[r], yr = a.re[r | bj], yi = a.im[r | bj];
-    const bool on = cm == 0 || (((gbase | goff<A>(mp, r)) & cm) == cv);
+    const bool on = pred_on<A>(p, g, r);
     a.re[r] = on ? yr : xr;
     a.im[r] = on ? yi : xi;
     a.re[r | bj] = on ? xr : yr;
@@ -181,10 +251,27 @@ __device__ __forceinline__ void op_x1(Amp<R, 1 << A> &a, const SweepOp<R> &op, u
   }
 }
 
+template <int A, typename R>
+__device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                           const Ctx &c, const Regs<A> &g) {
+  const int k = (int)op.s0 - (int)kLaneSlot;
+  const int addr = (c.lane ^ (1 << k)) << 2;
+  const Pred p = make_pred<A>(op, c, g);
+#pragma unroll
+  for (int r = 0; r < (1 << A); ++r) {
+    const R xr = a.re[r], xi = a.im[r];
+    const R yr = xchg(xr, addr), yi = xchg(xi, addr);
+    const bool on = pred_on<A>(p, g, r);
+    a.re[r] = on ? yr : xr;
+    a.im[r] = on ? yi : xi;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <int J0, int J1, int A, typename R>
-__device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const SweepOp<R> &op, uint64_t gbase,
-                                          const Map<A> &mp) {
-  const uint64_t cm = op.cmask, cv = op.cval;
+__device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                          const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+  const Pred p = make_pred<A>(op, c, g);
   constexpr int b0 = 1 << J0, b1 = 1 << J1;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
@@ -197,12 +284,11 @@ __device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const SweepOp<R> &o
     for (int i = 0; i < 4; ++i) {
       R sr = 0, si = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        cfma(sr, si, op.m[8 * i + 2 * j], op.m[8 * i + 2 * j + 1], xr[j], xi[j]);
+      for (int j = 0; j < 4; ++j) cfma(sr, si, m[8 * i + 2 * j], m[8 * i + 2 * j + 1], xr[j], xi[j]);
       yr[i] = sr;
       yi[i] = si;
     }
-    const bool on = cm == 0 || (((gbase | goff<A>(mp, r)) & cm) == cv);
+    const bool on = pred_on<A>(p, g, r);
     a.re[i0] = on ? yr[0] : xr[0];
     a.im[i0] = on ? yi[0] : xi[0];
     a.re[i1] = on ? yr[1] : xr[1];
@@ -215,13 +301,13 @@ __device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const SweepOp<R> &o
 }
 
 template <int A, typename R>
-__device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const SweepOp<R> &op, uint64_t gbase,
-                                          const Map<A> &mp) {
-  const R mr = op.m[0], mi = op.m[1];
-  const uint64_t cm = op.cmask, cv = op.cval;
+__device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                          const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+  const R mr = m[0], mi = m[1];
+  const Pred p = make_pred<A>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
-    const bool on = ((gbase | goff<A>(mp, r)) & cm) == cv;
+    const bool on = pred_on<A>(p, g, r);
     const R xr = a.re[r], xi = a.im[r];
     R sr = 0, si = 0;
     cfma(sr, si, mr, mi, xr, xi);
@@ -231,166 +317,141 @@ __device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const SweepOp<R> &o
 }
 
 template <int A, typename R>
-__device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const SweepOp<R> &op, uint64_t gbase,
-                                        const Map<A> &mp) {
-  const uint64_t cm = op.cmask, cv = op.cval;
+__device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                        const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+  const Pred p = make_pred<A>(op, c, g);
   const int na = op.na;
-  if (na == 1) {
-    const int q = op.act[0];
-    const R d0r = op.m[0], d0i = op.m[1], d1r = op.m[2], d1i = op.m[3];
-#pragma unroll
-    for (int r = 0; r < (1 << A); ++r) {
-      const uint64_t g = gbase | goff<A>(mp, r);
-      const bool on = (g & cm) == cv;
-      const bool hi = (g >> q) & 1ull;
-      const R xr = a.re[r], xi = a.im[r];
-      R sr = 0, si = 0;
-      cfma(sr, si, hi ? d1r : d0r, hi ? d1i : d0i, xr, xi);
-      a.re[r] = on ? sr : xr;
-      a.im[r] = on ? si : xi;
-    }
-    return;
-  }
+  const int q0 = op.act[0], q1 = na > 1 ? op.act[1] : op.act[0];
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
-    const uint64_t g = gbase | goff<A>(mp, r);
-    int pat = 0;
-    for (int j = 0; j < na; ++j) pat |= (int)((g >> op.act[j]) & 1ull) << j;
-    const bool on = (g & cm) == cv;
+    const uint64_t gi = c.gbase | goff<A>(g, r);
+    const bool b0 = (gi >> q0) & 1ull;
+    const bool b1 = na > 1 && ((gi >> q1) & 1ull);
+    const R vr = b1 ? (b0 ? m[6] : m[4]) : (b0 ? m[2] : m[0]);
+    const R vi = b1 ? (b0 ? m[7] : m[5]) : (b0 ? m[3] : m[1]);
+    const bool on = pred_on<A>(p, g, r);
     const R xr = a.re[r], xi = a.im[r];
     R sr = 0, si = 0;
-    cfma(sr, si, op.m[2 * pat], op.m[2 * pat + 1], xr, xi);
+    cfma(sr, si, vr, vi, xr, xi);
     a.re[r] = on ? sr : xr;
     a.im[r] = on ? si : xi;
   }
 }
 
-template <int A, typename R>
-__device__ __forceinline__ void dispatch_dense1(Amp<R, 1 << A> &a, const SweepOp<R> &op,
-                                                uint64_t gbase, const Map<A> &mp) {
+template <bool kCtl, int A, typename R>
+__device__ __forceinline__ void dispatch_1q(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                            const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
   switch (op.s0) {
-    case 0: op_dense1<0, A>(a, op, gbase, mp); break;
-    case 1: if constexpr (A > 1) op_dense1<1, A>(a, op, gbase, mp); break;
-    case 2: if constexpr (A > 2) op_dense1<2, A>(a, op, gbase, mp); break;
-    case 3: if constexpr (A > 3) op_dense1<3, A>(a, op, gbase, mp); break;
-    case 4: if constexpr (A > 4) op_dense1<4, A>(a, op, gbase, mp); break;
-    default: break;
+    case 0: op_1q_reg<0, kCtl, A>(a, op, m, c, g); break;
+    case 1: if constexpr (A > 1) op_1q_reg<1, kCtl, A>(a, op, m, c, g); break;
+    case 2: if constexpr (A > 2) op_1q_reg<2, kCtl, A>(a, op, m, c, g); break;
+    case 3: if constexpr (A > 3) op_1q_reg<3, kCtl, A>(a, op, m, c, g); break;
+    case 4: if constexpr (A > 4) op_1q_reg<4, kCtl, A>(a, op, m, c, g); break;
+    default: op_1q_lane<kCtl, A>(a, op, m, c, g); break;
   }
 }
 
 template <int A, typename R>
-__device__ __forceinline__ void dispatch_x1(Amp<R, 1 << A> &a, const SweepOp<R> &op,
-                                            uint64_t gbase, const Map<A> &mp) {
+__device__ __forceinline__ void dispatch_x1(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                            const Ctx &c, const Regs<A> &g) {
   switch (op.s0) {
-    case 0: op_x1<0, A>(a, op, gbase, mp); break;
-    case 1: if constexpr (A > 1) op_x1<1, A>(a, op, gbase, mp); break;
-    case 2: if constexpr (A > 2) op_x1<2, A>(a, op, gbase, mp); break;
-    case 3: if constexpr (A > 3) op_x1<3, A>(a, op, gbase, mp); break;
-    case 4: if constexpr (A > 4) op_x1<4, A>(a, op, gbase, mp); break;
-    default: break;
+    case 0: op_x1_reg<0, A, R>(a, op, c, g); break;
+    case 1: if constexpr (A > 1) op_x1_reg<1, A, R>(a, op, c, g); break;
+    case 2: if constexpr (A > 2) op_x1_reg<2, A, R>(a, op, c, g); break;
+    case 3: if constexpr (A > 3) op_x1_reg<3, A, R>(a, op, c, g); break;
+    case 4: if constexpr (A > 4) op_x1_reg<4, A, R>(a, op, c, g); break;
+    default: op_x1_lane<A, R>(a, op, c, g); break;
   }
 }
 
 template <int J0, int A, typename R>
-__device__ __forceinline__ void dispatch_dense2_b(Amp<R, 1 << A> &a, const SweepOp<R> &op,
-                                                  uint64_t gbase, const Map<A> &mp) {
+__device__ __forceinline__ void dispatch_dense2_b(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                                  const QS_CONST R *m, const Ctx &c,
+                                                  const Regs<A> &g) {
   switch (op.s1) {
-    case 0: if constexpr (J0 != 0) op_dense2<J0, 0, A>(a, op, gbase, mp); break;
-    case 1: if constexpr (A > 1 && J0 != 1) op_dense2<J0, 1, A>(a, op, gbase, mp); break;
-    case 2: if constexpr (A > 2 && J0 != 2) op_dense2<J0, 2, A>(a, op, gbase, mp); break;
-    case 3: if constexpr (A > 3 && J0 != 3) op_dense2<J0, 3, A>(a, op, gbase, mp); break;
-    case 4: if constexpr (A > 4 && J0 != 4) op_dense2<J0, 4, A>(a, op, gbase, mp); break;
+    case 0: if constexpr (J0 != 0) op_dense2<J0, 0, A>(a, op, m, c, g); break;
+    case 1: if constexpr (A > 1 && J0 != 1) op_dense2<J0, 1, A>(a, op, m, c, g); break;
+    case 2: if constexpr (A > 2 && J0 != 2) op_dense2<J0, 2, A>(a, op, m, c, g); break;
+    case 3: if constexpr (A > 3 && J0 != 3) op_dense2<J0, 3, A>(a, op, m, c, g); break;
+    case 4: if constexpr (A > 4 && J0 != 4) op_dense2<J0, 4, A>(a, op, m, c, g); break;
     default: break;
   }
 }
 
 template <int A, typename R>
-__device__ __forceinline__ void dispatch_dense2(Amp<R, 1 << A> &a, const SweepOp<R> &op,
-                                                uint64_t gbase, const Map<A> &mp) {
+__device__ __forceinline__ void dispatch_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                                const QS_CONST R *m, const Ctx &c,
+                                                const Regs<A> &g) {
   switch (op.s0) {
-    case 0: dispatch_dense2_b<0, A>(a, op, gbase, mp); break;
-    case 1: if constexpr (A > 1) dispatch_dense2_b<1, A>(a, op, gbase, mp); break;
-    case 2: if constexpr (A > 2) dispatch_dense2_b<2, A>(a, op, gbase, mp); break;
-    case 3: if constexpr (A > 3) dispatch_dense2_b<3, A>(a, op, gbase, mp); break;
-    case 4: if constexpr (A > 4) dispatch_dense2_b<4, A>(a, op, gbase, mp); break;
+    case 0: dispatch_dense2_b<0, A>(a, op, m, c, g); break;
+    case 1: if constexpr (A > 1) dispatch_dense2_b<1, A>(a, op, m, c, g); break;
+    case 2: if constexpr (A > 2) dispatch_dense2_b<2, A>(a, op, m, c, g); break;
+    case 3: if constexpr (A > 3) dispatch_dense2_b<3, A>(a, op, m, c, g); break;
+    case 4: if constexpr (A > 4) dispatch_dense2_b<4, A>(a, op, m, c, g); break;
     default: break;
   }
 }
 
-template <int A, typename R>
-__device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const SweepOp<R> *__restrict__ ops,
-                                        int begin, int end, uint64_t gbase, const Map<A> &mp) {
-  for (int o = begin; o < end; ++o) {
-    const SweepOp<R> &op = ops[o];
-    switch (op.type) {
-      case OP_DENSE1: dispatch_dense1<A>(a, op, gbase, mp); break;
-      case OP_X1: dispatch_x1<A>(a, op, gbase, mp); break;
-      case OP_DENSE2: dispatch_dense2<A>(a, op, gbase, mp); break;
-      case OP_SPARSE: op_sparse<A>(a, op, gbase, mp); break;
-      case OP_DIAG: op_diag<A>(a, op, gbase, mp); break;
-      default: break;
-    }
-  }
-}
-
-// One workgroup per tile.  Every tile's amplitudes are read from HBM once
-// and written once; all phases in between run in registers and LDS.
-template <typename R, int T>
+// One wave per tile: load, apply every op, store.
+template <typename R, int A>
 __global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restrict__ psi,
-                                                    const SweepHeader *__restrict__ hdr,
-                                                    const SweepOp<R> *__restrict__ ops) {
+                                                    const char *__restrict__ blob_g) {
   using V = typename SV2<R>::T;
-  constexpr int A = T - 8;
+  constexpr int T = kLaneBits + A;
   constexpr int NA = 1 << A;
-  const SweepHeader &h = *hdr;
-  extern __shared__ unsigned char lds_raw[];
-  V *lds = reinterpret_cast<V *>(lds_raw);
-  const int t = threadIdx.x;
-  const uint64_t base = sw_expand((uint64_t)blockIdx.x, h.tile);
+  const QS_CONST char *blob = (const QS_CONST char *)blob_g;
+  const QS_CONST SweepHeader &h = *(const QS_CONST SweepHeader *)blob;
+  const size_t hb = (sizeof(SweepHeader) + 15) & ~(size_t)15;
+  const QS_CONST OpRec *ops = (const QS_CONST OpRec *)(blob + hb);
+  const QS_CONST R *pool =
+      (const QS_CONST R *)(blob + hb + (size_t)h.nops * sizeof(OpRec));
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / kWave) + (threadIdx.x / kWave);
+  if (tile >= h.ntiles) return;  // whole waves only
+  Regs<A> g;
+  g.mask = 0;
+#pragma unroll
+  for (int j = 0; j < A; ++j) {
+    g.gb[j] = h.gb[j];
+    g.mask |= g.gb[j];
+  }
+  uint64_t lo = 0;
+#pragma unroll
+  for (int i = 0; i < kLaneBits; ++i) lo |= ((lane >> i) & 1) ? h.lg[i] : 0ull;
+  Ctx c;
+  c.gbase = tile_base<T>(tile, h) | lo;
+  c.lane = lane;
   Amp<R, NA> a;
-  Map<A> mp;
-  make_map<T, A>(h, 0, t, mp);
   {
-    const uint64_t gb = base | mp.gthr;
     V v[NA];
 #pragma unroll
-    for (int r = 0; r < NA; ++r) v[r] = psi[gb | goff<A>(mp, r)];
+    for (int r = 0; r < NA; ++r) v[r] = psi[c.gbase | goff<A>(g, r)];
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
       a.re[r] = v[r].x;
       a.im[r] = v[r].y;
     }
   }
-  run_ops<A>(a, ops, h.ph[0].op_begin, h.ph[0].op_end, base | mp.gthr, mp);
-  const int nphase = h.nphase;
-  for (int p = 1; p < nphase; ++p) {
-#pragma unroll
-    for (int r = 0; r < NA; ++r) {
-      V v;
-      v.x = a.re[r];
-      v.y = a.im[r];
-      lds[swz(mp.lthr | loff<A>(mp, r))] = v;
+  const int nops = (int)h.nops;
+  for (int o = 0; o < nops; ++o) {
+    const QS_CONST OpRec &op = ops[o];
+    const QS_CONST R *m = pool + op.moff;
+    switch (op.type) {
+      case OP_U1: dispatch_1q<false, A>(a, op, m, c, g); break;
+      case OP_C1: dispatch_1q<true, A>(a, op, m, c, g); break;
+      case OP_X1: dispatch_x1<A>(a, op, c, g); break;
+      case OP_DENSE2: dispatch_dense2<A>(a, op, m, c, g); break;
+      case OP_SPARSE: op_sparse<A>(a, op, m, c, g); break;
+      case OP_DIAG: op_diag<A>(a, op, m, c, g); break;
+      default: break;
     }
-    __syncthreads();
-    make_map<T, A>(h, p, t, mp);
-#pragma unroll
-    for (int r = 0; r < NA; ++r) {
-      const V v = lds[swz(mp.lthr | loff<A>(mp, r))];
-      a.re[r] = v.x;
-      a.im[r] = v.y;
-    }
-    __syncthreads();
-    run_ops<A>(a, ops, h.ph[p].op_begin, h.ph[p].op_end, base | mp.gthr, mp);
   }
-  {
-    const uint64_t gb = base | mp.gthr;
 #pragma unroll
-    for (int r = 0; r < NA; ++r) {
-      V v;
-      v.x = a.re[r];
-      v.y = a.im[r];
-      psi[gb | goff<A>(mp, r)] = v;
-    }
+  for (int r = 0; r < NA; ++r) {
+    V v;
+    v.x = a.re[r];
+    v.y = a.im[r];
+    psi[c.gbase | goff<A>(g, r)] = v;
   }
 }
 
