@@ -47,6 +47,20 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def recorded_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (calibrated against copy kernels of known size) for this workload, as
+    recorded in profiles/<name>; None if absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", text)
+    return int(m.group(1)) if m else None
+
+
 def timing(reg: Register, reset: bool = True):
     import ctypes
     ms = (ctypes.c_double * 8)()
@@ -194,7 +208,21 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: a 1 GiB torch copy (counter calibration), "
+                         "then one warmup-free step in complex128 only")
     args = ap.parse_args()
+    if args.profile:
+        args.steps, args.warmup = 1, 0
+        args.no_c64 = args.no_unfused = args.no_cpu = True
+        args.adjoint = False
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("profile: calibration copy of 1 GiB done")
+        del src, dst
     if args.gpus != 1:
         print("bench.py runs on one GPU: the state-vector path is built and measured on a "
               "single MI355X (multi-GPU qubit sharding is design only); use --gpus 1",
@@ -251,7 +279,10 @@ def main() -> int:
                    "qubits": n, "layers": args.layers, "gates_per_step": G,
                    "state_bytes": (1 << n) * amp, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": (recorded_traffic("k_sweep_c128_30q.md")
+                                 if n == 30 and amp == 16 else None),
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/k_sweep_c128_30q.md",
                      "kernel": "k_sweep (fused sweep, complex128)",
                      "launches": cnt[0], "avg_ms": sweep_avg_s * 1e3,
                      "bytes_per_launch": sweep_bytes},
