@@ -24,7 +24,7 @@
 #include <vector>
 
 #include "qs_sweep.h"
-#include "qs_sweep_device.h"
+#include "qs_sweep_kernel.h"
 
 namespace qs {
 namespace {
@@ -179,8 +179,12 @@ hipError_t launch_sweep(SweepRunner &run, const void *d_blob, uint64_t ntiles) {
   using V = typename SV2<R>::T;
   const uint64_t per_block = kThreads / kWave;
   const uint64_t grid = (ntiles + per_block - 1) / per_block;
-  hipLaunchKernelGGL((k_sweep<R, A>), dim3((unsigned)grid), dim3(kThreads), 0, run.stream,
-                     (V *)run.psi, (const char *)d_blob);
+  if (run.n <= 32)
+    hipLaunchKernelGGL((k_sweep<R, A, uint32_t>), dim3((unsigned)grid), dim3(kThreads), 0,
+                       run.stream, (V *)run.psi, (const char *)d_blob);
+  else
+    hipLaunchKernelGGL((k_sweep<R, A, uint64_t>), dim3((unsigned)grid), dim3(kThreads), 0,
+                       run.stream, (V *)run.psi, (const char *)d_blob);
   return hipGetLastError();
 }
 

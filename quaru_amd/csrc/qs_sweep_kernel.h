@@ -20,6 +20,10 @@
 //
 // The descriptor is read through the constant address space with 32/64-bit
 // fields only, so the compiler keeps every descriptor read on scalar loads.
+// Index arithmetic runs in 32 bits when the state has at most 2^32
+// amplitudes (I = uint32_t) and in 64 bits otherwise.  The state is streamed
+// with nontemporal loads and stores: every amplitude is touched once per
+// sweep, so caching it would only evict other data.
 // Amplitudes live in per-lane register arrays re[]/im[] of plain scalars
 // (complex structs selected as a whole would be spilled to scratch).
 #pragma once
@@ -76,6 +80,12 @@ __host__ __device__ inline size_t blob_bytes(int nops, int npool) {
   return h + o + p;
 }
 
+// Native two-element vectors for the nontemporal builtins.
+template <typename R>
+struct SNT {
+  typedef R T __attribute__((ext_vector_type(2)));
+};
+
 template <typename R>
 struct SV2;
 template <>
@@ -98,46 +108,48 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t p, const QS_CONST SweepHe
   return p;
 }
 
-template <int A>
+template <int A, typename I>
 struct Regs {
-  uint64_t gb[A];  // global bit of register slot j (wave-uniform)
-  uint64_t mask;   // OR of gb[]
+  I gb[A];  // global bit of register slot j (wave-uniform)
+  I mask;   // OR of gb[]
 };
 
-template <int A>
-__device__ __forceinline__ uint64_t goff(const Regs<A> &g, int r) {
-  uint64_t o = 0;
+template <int A, typename I>
+__device__ __forceinline__ I goff(const Regs<A, I> &g, int r) {
+  I o = 0;
 #pragma unroll
   for (int j = 0; j < A; ++j)
     if ((r >> j) & 1) o |= g.gb[j];
   return o;
 }
 
+template <typename I>
 struct Ctx {
-  uint64_t gbase;  // tile base | this lane's lane-bit offsets (no register bits)
+  I gbase;  // tile base | this lane's lane-bit offsets (no register bits)
   int lane;
 };
 
 // on(r) = lane_ok && ((goff(r) & cmr) == cvr): the second factor is uniform.
+template <typename I>
 struct Pred {
   bool lane_ok;
-  uint64_t cmr, cvr;
+  I cmr, cvr;
 };
 
-template <int A>
-__device__ __forceinline__ Pred make_pred(const QS_CONST OpRec &op, const Ctx &c,
-                                          const Regs<A> &g) {
-  const uint64_t cm = op.cmask, cv = op.cval;
-  const uint64_t cmo = cm & ~g.mask;
-  Pred p;
+template <int A, typename I>
+__device__ __forceinline__ Pred<I> make_pred(const QS_CONST OpRec &op, const Ctx<I> &c,
+                                             const Regs<A, I> &g) {
+  const I cm = (I)op.cmask, cv = (I)op.cval;
+  const I cmo = cm & ~g.mask;
+  Pred<I> p;
   p.cmr = cm & g.mask;
   p.cvr = cv & g.mask;
   p.lane_ok = (c.gbase & cmo) == (cv & cmo);
   return p;
 }
 
-template <int A>
-__device__ __forceinline__ bool pred_on(const Pred &p, const Regs<A> &g, int r) {
+template <int A, typename I>
+__device__ __forceinline__ bool pred_on(const Pred<I> &p, const Regs<A, I> &g, int r) {
   return p.lane_ok && ((goff<A>(g, r) & p.cmr) == p.cvr);
 }
 
@@ -169,12 +181,12 @@ __device__ __forceinline__ void cfma(R &ore, R &oim, R mr, R mi, R xr, R xi) {
 }
 
 // 2x2 on register slot J; kCtl selects the controlled form.
-template <int J, bool kCtl, int A, typename R>
+template <int J, bool kCtl, int A, typename R, typename I>
 __device__ __forceinline__ void op_1q_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                          const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+                                          const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
   const R m0r = m[0], m0i = m[1], m1r = m[2], m1i = m[3];
   const R m2r = m[4], m2i = m[5], m3r = m[6], m3i = m[7];
-  Pred p{};
+  Pred<I> p{};
   if constexpr (kCtl) p = make_pred<A>(op, c, g);
   constexpr int bj = 1 << J;
 #pragma unroll
@@ -204,15 +216,15 @@ __device__ __forceinline__ void op_1q_reg(Amp<R, 1 << A> &a, const QS_CONST OpRe
 // 2x2 on a lane qubit: a lane with the bit clear holds x and gets the
 // partner's y: out = m00 x + m01 y; a lane with the bit set holds y and
 // computes out = m10 x + m11 y from the partner's x.
-template <bool kCtl, int A, typename R>
+template <bool kCtl, int A, typename R, typename I>
 __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                           const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+                                           const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
   const int k = (int)op.s0 - (int)kLaneSlot;
   const bool hi = (c.lane >> k) & 1;
   const int addr = (c.lane ^ (1 << k)) << 2;
   const R sr = hi ? m[6] : m[0], si = hi ? m[7] : m[1];
   const R pr = hi ? m[4] : m[2], pi = hi ? m[5] : m[3];
-  Pred p{};
+  Pred<I> p{};
   if constexpr (kCtl) p = make_pred<A>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
@@ -234,10 +246,10 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
   }
 }
 
-template <int J, int A, typename R>
+template <int J, int A, typename R, typename I>
 __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                          const Ctx &c, const Regs<A> &g) {
-  const Pred p = make_pred<A>(op, c, g);
+                                          const Ctx<I> &c, const Regs<A, I> &g) {
+  const Pred<I> p = make_pred<A>(op, c, g);
   constexpr int bj = 1 << J;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
@@ -251,12 +263,12 @@ __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRe
   }
 }
 
-template <int A, typename R>
+template <int A, typename R, typename I>
 __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                           const Ctx &c, const Regs<A> &g) {
+                                           const Ctx<I> &c, const Regs<A, I> &g) {
   const int k = (int)op.s0 - (int)kLaneSlot;
   const int addr = (c.lane ^ (1 << k)) << 2;
-  const Pred p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const R xr = a.re[r], xi = a.im[r];
@@ -268,10 +280,10 @@ __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
   }
 }
 
-template <int J0, int J1, int A, typename R>
+template <int J0, int J1, int A, typename R, typename I>
 __device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                          const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
-  const Pred p = make_pred<A>(op, c, g);
+                                          const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
+  const Pred<I> p = make_pred<A>(op, c, g);
   constexpr int b0 = 1 << J0, b1 = 1 << J1;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
@@ -300,11 +312,11 @@ __device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRe
   }
 }
 
-template <int A, typename R>
+template <int A, typename R, typename I>
 __device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                          const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+                                          const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
   const R mr = m[0], mi = m[1];
-  const Pred p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const bool on = pred_on<A>(p, g, r);
@@ -316,17 +328,17 @@ __device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const QS_CONST OpRe
   }
 }
 
-template <int A, typename R>
+template <int A, typename R, typename I>
 __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                        const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
-  const Pred p = make_pred<A>(op, c, g);
+                                        const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
+  const Pred<I> p = make_pred<A>(op, c, g);
   const int na = op.na;
   const int q0 = op.act[0], q1 = na > 1 ? op.act[1] : op.act[0];
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
-    const uint64_t gi = c.gbase | goff<A>(g, r);
-    const bool b0 = (gi >> q0) & 1ull;
-    const bool b1 = na > 1 && ((gi >> q1) & 1ull);
+    const I gi = c.gbase | goff<A>(g, r);
+    const bool b0 = (gi >> q0) & 1u;
+    const bool b1 = na > 1 && ((gi >> q1) & 1u);
     const R vr = b1 ? (b0 ? m[6] : m[4]) : (b0 ? m[2] : m[0]);
     const R vi = b1 ? (b0 ? m[7] : m[5]) : (b0 ? m[3] : m[1]);
     const bool on = pred_on<A>(p, g, r);
@@ -338,9 +350,9 @@ __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec 
   }
 }
 
-template <bool kCtl, int A, typename R>
+template <bool kCtl, int A, typename R, typename I>
 __device__ __forceinline__ void dispatch_1q(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                            const QS_CONST R *m, const Ctx &c, const Regs<A> &g) {
+                                            const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
   switch (op.s0) {
     case 0: op_1q_reg<0, kCtl, A>(a, op, m, c, g); break;
     case 1: if constexpr (A > 1) op_1q_reg<1, kCtl, A>(a, op, m, c, g); break;
@@ -351,23 +363,23 @@ __device__ __forceinline__ void dispatch_1q(Amp<R, 1 << A> &a, const QS_CONST Op
   }
 }
 
-template <int A, typename R>
+template <int A, typename R, typename I>
 __device__ __forceinline__ void dispatch_x1(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                            const Ctx &c, const Regs<A> &g) {
+                                            const Ctx<I> &c, const Regs<A, I> &g) {
   switch (op.s0) {
-    case 0: op_x1_reg<0, A, R>(a, op, c, g); break;
-    case 1: if constexpr (A > 1) op_x1_reg<1, A, R>(a, op, c, g); break;
-    case 2: if constexpr (A > 2) op_x1_reg<2, A, R>(a, op, c, g); break;
-    case 3: if constexpr (A > 3) op_x1_reg<3, A, R>(a, op, c, g); break;
-    case 4: if constexpr (A > 4) op_x1_reg<4, A, R>(a, op, c, g); break;
-    default: op_x1_lane<A, R>(a, op, c, g); break;
+    case 0: op_x1_reg<0, A, R, I>(a, op, c, g); break;
+    case 1: if constexpr (A > 1) op_x1_reg<1, A, R, I>(a, op, c, g); break;
+    case 2: if constexpr (A > 2) op_x1_reg<2, A, R, I>(a, op, c, g); break;
+    case 3: if constexpr (A > 3) op_x1_reg<3, A, R, I>(a, op, c, g); break;
+    case 4: if constexpr (A > 4) op_x1_reg<4, A, R, I>(a, op, c, g); break;
+    default: op_x1_lane<A, R, I>(a, op, c, g); break;
   }
 }
 
-template <int J0, int A, typename R>
+template <int J0, int A, typename R, typename I>
 __device__ __forceinline__ void dispatch_dense2_b(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                                  const QS_CONST R *m, const Ctx &c,
-                                                  const Regs<A> &g) {
+                                                  const QS_CONST R *m, const Ctx<I> &c,
+                                                  const Regs<A, I> &g) {
   switch (op.s1) {
     case 0: if constexpr (J0 != 0) op_dense2<J0, 0, A>(a, op, m, c, g); break;
     case 1: if constexpr (A > 1 && J0 != 1) op_dense2<J0, 1, A>(a, op, m, c, g); break;
@@ -378,10 +390,10 @@ __device__ __forceinline__ void dispatch_dense2_b(Amp<R, 1 << A> &a, const QS_CO
   }
 }
 
-template <int A, typename R>
+template <int A, typename R, typename I>
 __device__ __forceinline__ void dispatch_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                                const QS_CONST R *m, const Ctx &c,
-                                                const Regs<A> &g) {
+                                                const QS_CONST R *m, const Ctx<I> &c,
+                                                const Regs<A, I> &g) {
   switch (op.s0) {
     case 0: dispatch_dense2_b<0, A>(a, op, m, c, g); break;
     case 1: if constexpr (A > 1) dispatch_dense2_b<1, A>(a, op, m, c, g); break;
@@ -393,7 +405,7 @@ __device__ __forceinline__ void dispatch_dense2(Amp<R, 1 << A> &a, const QS_CONS
 }
 
 // One wave per tile: load, apply every op, store.
-template <typename R, int A>
+template <typename R, int A, typename I>
 __global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restrict__ psi,
                                                     const char *__restrict__ blob_g) {
   using V = typename SV2<R>::T;
@@ -408,24 +420,25 @@ __global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restri
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / kWave) + (threadIdx.x / kWave);
   if (tile >= h.ntiles) return;  // whole waves only
-  Regs<A> g;
+  Regs<A, I> g;
   g.mask = 0;
 #pragma unroll
   for (int j = 0; j < A; ++j) {
-    g.gb[j] = h.gb[j];
+    g.gb[j] = (I)h.gb[j];
     g.mask |= g.gb[j];
   }
-  uint64_t lo = 0;
+  I lo = 0;
 #pragma unroll
-  for (int i = 0; i < kLaneBits; ++i) lo |= ((lane >> i) & 1) ? h.lg[i] : 0ull;
-  Ctx c;
-  c.gbase = tile_base<T>(tile, h) | lo;
+  for (int i = 0; i < kLaneBits; ++i) lo |= ((lane >> i) & 1) ? (I)h.lg[i] : (I)0;
+  Ctx<I> c;
+  c.gbase = (I)tile_base<T>(tile, h) | lo;
   c.lane = lane;
   Amp<R, NA> a;
   {
-    V v[NA];
+    typename SNT<R>::T v[NA];
+    const typename SNT<R>::T *src = reinterpret_cast<const typename SNT<R>::T *>(psi);
 #pragma unroll
-    for (int r = 0; r < NA; ++r) v[r] = psi[c.gbase | goff<A>(g, r)];
+    for (int r = 0; r < NA; ++r) v[r] = __builtin_nontemporal_load(src + (c.gbase | goff<A>(g, r)));
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
       a.re[r] = v[r].x;
@@ -446,12 +459,13 @@ __global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restri
       default: break;
     }
   }
+  typename SNT<R>::T *dst = reinterpret_cast<typename SNT<R>::T *>(psi);
 #pragma unroll
   for (int r = 0; r < NA; ++r) {
-    V v;
+    typename SNT<R>::T v;
     v.x = a.re[r];
     v.y = a.im[r];
-    psi[c.gbase | goff<A>(g, r)] = v;
+    __builtin_nontemporal_store(v, dst + (c.gbase | goff<A>(g, r)));
   }
 }
 
