@@ -146,6 +146,12 @@ int qs_sample_cdf(qs_register *r, const double *uniforms, size_t count, uint64_t
 int qs_get_state(qs_register *r, double *out);
 int qs_set_state(qs_register *r, const double *in);
 
+/* `state[indices[i]]` for i < count as complex128 into out[2i], out[2i+1]:
+ * the element reads of the `state` field (examples/grovers.rs and the
+ * examples' tests index it) without copying 2^n amplitudes to the host.
+ * An index >= 2^n -> QS_ERR_ARGUMENT. */
+int qs_get_amplitudes(qs_register *r, const uint64_t *indices, size_t count, double *out);
+
 /* sum_i |psi_i|^2 in FP64. */
 int qs_norm_sqr(qs_register *r, double *out);
 

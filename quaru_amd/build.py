@@ -73,7 +73,41 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
         os.replace(tmp, LIB)
     finally:
         shutil.rmtree(objdir, ignore_errors=True)
+    build_programs(verbose=verbose)
     return LIB
+
+
+# C++ programs over the host API (include/quaru/quaru.hpp): the reference's
+# integration tests and examples, linked against the library above.
+CPP_DIR = os.path.join(HERE, "cpp")
+BIN_DIR = os.path.join(HERE, "bin")
+PROGRAMS = ["register_operation_tests", "examples"]
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_programs(verbose: bool = False) -> list:
+    """g++ each program in ``quaru_amd/cpp`` into ``quaru_amd/bin``; the
+    binaries find the library through an rpath relative to themselves."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    procs = []
+    for name in PROGRAMS:
+        out = os.path.join(BIN_DIR, name)
+        cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", INCLUDE,
+               os.path.join(CPP_DIR, name + ".cpp"), "-L", HERE, "-lquaru_hip",
+               "-Wl,-rpath,$ORIGIN/..", "-o", out]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                             stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for name, p in procs:
+        log, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append((name, log))
+    if failed:
+        raise RuntimeError("g++ failed:\n" + "\n".join(f"--- {n}\n{o}" for n, o in failed))
+    return [os.path.join(BIN_DIR, n) for n in PROGRAMS]
 
 
 if __name__ == "__main__":

@@ -115,23 +115,18 @@ def random_circuit(n: int, layers: int, seed: int, triples: int = 0,
     return w
 
 
-def grover_ops(n: int, iterations: int, seed: int):
-    """Grover on n qubits: H on all, then `iterations` x [phase flip of the
-    seeded winner w, diffusion H^n . diag(1, -1, ..., -1) . H^n].
-
-    Returns (winner, list of steps); a step is ("all", 2x2) for apply_all,
-    ("flip", index) for the -1 at one basis index, and ("diffusion_diag",)
-    for diag(1, -1, ..., -1) (examples/grovers.rs:193-199, 208-218)."""
-    rng = SplitMix64(seed)
-    w = rng.below(1 << n)
-    h = ops.hadamard(0).matrix()
-    steps = [("all", h)]
-    for _ in range(iterations):
-        steps.append(("flip", w))
-        steps.append(("all", h))
-        steps.append(("diffusion_diag",))
-        steps.append(("all", h))
-    return w, steps
+def inverse(opl: List[ops.Operation]) -> List[ops.Operation]:
+    """The adjoint circuit: the conjugate transposes in reverse order, so that
+    running opl and then inverse(opl) is the identity (a mirror circuit)."""
+    out = []
+    for op in reversed(opl):
+        m = op.active_matrix.conj().T
+        if op.is_structured:
+            out.append(ops.Operation.controlled(op.controls, m, op.active_targets,
+                                                control_values=op.control_values))
+        else:
+            out.append(ops.Operation(m, op.active_targets))
+    return out
 
 
 def grover_closed_form(n: int, t: int, w: int):
@@ -142,6 +137,47 @@ def grover_closed_form(n: int, t: int, w: int):
     aw = math.sin((2 * t + 1) * th)
     ao = math.cos((2 * t + 1) * th) / math.sqrt(N - 1.0)
     return aw, ao
+
+
+def hadamard_layer(n: int, fused: bool, block: int = 5) -> List[ops.Operation]:
+    """H on every qubit: n single-qubit gates, or (fused) dense Hadamard
+    transforms on consecutive blocks of up to `block` qubits (32 x 32
+    unitaries for block = 5)."""
+    if not fused:
+        return [ops.hadamard(q) for q in range(n)]
+    return [ops.hadamard_transform(list(range(q, min(q + block, n))))
+            for q in range(0, n, block)]
+
+
+def phase_flip(n: int, index: int) -> ops.Operation:
+    """-1 on basis state `index` only: diag on qubit 0 under controls on
+    qubits 1..n-1 matching the other bits of `index`."""
+    if n == 1:
+        m = np.diag([-1.0, 1.0]) if index == 0 else np.diag([1.0, -1.0])
+        return ops.Operation(m.astype(np.complex128), [0], label="flip")
+    m = np.diag([1.0, -1.0]) if index & 1 else np.diag([-1.0, 1.0])
+    controls = list(range(1, n))
+    cvals = [bool((index >> q) & 1) for q in controls]
+    return ops.Operation.controlled(controls, m, [0], label="flip", control_values=cvals)
+
+
+def grover_operations(n: int, iterations: int, seed: int, fused: bool):
+    """The Grover workload as Operations: H on all qubits, then `iterations`
+    x [oracle: -1 on the seeded winner w; diffusion H^n . diag(1, -1, ..., -1)
+    . H^n], with the signs of examples/grovers.rs:184-219.  diag(1, -1, ...,
+    -1) is written as the global phase -1 (a 1-qubit -I) times a flip of
+    basis state 0, so the per-amplitude result (global phase included) equals
+    the dense matrices of the example.  Returns (w, operations)."""
+    w = SplitMix64(seed).below(1 << n)
+    minus_i = ops.Operation(-np.eye(2, dtype=np.complex128), [0], label="-I")
+    out = hadamard_layer(n, fused)
+    for _ in range(iterations):
+        out.append(phase_flip(n, w))
+        out += hadamard_layer(n, fused)
+        out.append(phase_flip(n, 0))
+        out.append(minus_i)
+        out += hadamard_layer(n, fused)
+    return w, out
 
 
 @dataclass

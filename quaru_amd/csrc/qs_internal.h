@@ -96,6 +96,10 @@ template <typename R>
 hipError_t launch_convert_from_c128(void *psi, const double *src, uint64_t count, hipStream_t s);
 template <typename R>
 hipError_t launch_convert_to_c128(double *dst, const void *psi, uint64_t count, hipStream_t s);
+// psi[d_idx[i]] widened to complex128 into dst[2i], dst[2i+1]; every index < 2^n.
+template <typename R>
+hipError_t launch_gather_amps(double *dst, const void *psi, const uint64_t *d_idx, uint64_t count,
+                              hipStream_t s);
 // Histogram of |psi|^2 over the bit pattern of m qubits: bin bit j <->
 // qubit pos_order[j]; fx holds those m positions fixed (set = 0).  d_part
 // holds blocks x 2^m partials.

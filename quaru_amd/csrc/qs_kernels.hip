@@ -675,6 +675,28 @@ __global__ __launch_bounds__(kBlock) void k_to_c128(double *__restrict__ dst,
   }
 }
 
+// Amplitudes at `count` given indices, widened to complex128.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_gather_amps(double *__restrict__ dst,
+                                                        const typename V2<R>::T *__restrict__ psi,
+                                                        const uint64_t *__restrict__ idx,
+                                                        uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= count) return;
+  const auto v = psi[idx[i]];
+  dst[2 * i] = (double)v.x;
+  dst[2 * i + 1] = (double)v.y;
+}
+
+template <typename R>
+hipError_t launch_gather_amps(double *dst, const void *psi, const uint64_t *d_idx, uint64_t count,
+                              hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_amps<R>, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, dst,
+                     (const typename V2<R>::T *)psi, d_idx, count);
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_convert_from_c128(void *psi, const double *src, uint64_t count, hipStream_t s) {
   unsigned grid = grid_for(count, kBlock * 4);
@@ -844,6 +866,8 @@ hipError_t launch_chunk_sums(const void *psi, int n, uint64_t chunk, double *d_o
   template hipError_t launch_convert_from_c128<R>(void *, const double *, uint64_t,             \
                                                   hipStream_t);                                  \
   template hipError_t launch_convert_to_c128<R>(double *, const void *, uint64_t, hipStream_t); \
+  template hipError_t launch_gather_amps<R>(double *, const void *, const uint64_t *, uint64_t,  \
+                                            hipStream_t);                                       \
   template hipError_t launch_set_histogram<R>(const void *, int, const Fixed &, const uint8_t *, \
                                               int, double *, int, double *, hipStream_t);        \
   template hipError_t launch_gather_probs<R>(const void *, const Fixed &, uint64_t, double *,    \

@@ -1096,6 +1096,30 @@ int qs_set_state(qs_register *r, const double *in) {
   return QS_OK;
 }
 
+int qs_get_amplitudes(qs_register *r, const uint64_t *indices, size_t count, double *out) {
+  if (!r || (count && (!indices || !out))) return fail(QS_ERR_ARGUMENT, 0, 0, "NULL argument");
+  for (size_t i = 0; i < count; ++i)
+    if (indices[i] >= r->len)
+      return fail(QS_ERR_ARGUMENT, i, (size_t)indices[i], "qs_get_amplitudes: index >= 2^n");
+  int rc = flush(r);
+  if (rc || count == 0) return rc;
+  char *d = nullptr;
+  const size_t ib = count * sizeof(uint64_t), ob = count * 2 * sizeof(double);
+  QS_HIP(hipMalloc((void **)&d, ib + ob));
+  uint64_t *d_idx = (uint64_t *)d;
+  double *d_out = (double *)(d + ib);
+  hipError_t e = hipMemcpyAsync(d_idx, indices, ib, hipMemcpyHostToDevice, r->stream);
+  if (e == hipSuccess)
+    e = by_dtype(r, [&](auto z) {
+      return qs::launch_gather_amps<decltype(z)>(d_out, r->psi, d_idx, count, r->stream);
+    });
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, ob, hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "qs_get_amplitudes");
+  return QS_OK;
+}
+
 int qs_norm_sqr(qs_register *r, double *out) {
   if (!r || !out) return fail(QS_ERR_ARGUMENT, 0, 0, "NULL argument");
   int rc = flush(r);

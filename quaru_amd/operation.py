@@ -36,13 +36,17 @@ class Operation:
     ``targets + controls`` and ``matrix()`` is the dense equivalent.
     """
 
-    __slots__ = ("_matrix", "_targets", "_controls", "_label")
+    __slots__ = ("_matrix", "_targets", "_controls", "_cvals", "_label")
 
     def __init__(self, matrix: np.ndarray, targets: Sequence[int],
-                 controls: Sequence[int] = (), label: str = ""):
+                 controls: Sequence[int] = (), label: str = "",
+                 control_values: Optional[Sequence[bool]] = None):
         self._matrix = np.ascontiguousarray(matrix, dtype=np.complex128)
         self._targets = [int(t) for t in targets]
         self._controls = [int(c) for c in controls]
+        self._cvals = None if control_values is None else [bool(v) for v in control_values]
+        if self._cvals is not None and len(self._cvals) != len(self._controls):
+            raise ValueError("one control value per control")
         self._label = label
 
     # -- construction -------------------------------------------------------
@@ -58,13 +62,15 @@ class Operation:
 
     @staticmethod
     def controlled(controls: Sequence[int], matrix, targets: Sequence[int],
-                   label: str = "") -> "Operation":
-        """``matrix`` on ``targets``, applied only where every control is 1."""
+                   label: str = "",
+                   control_values: Optional[Sequence[bool]] = None) -> "Operation":
+        """``matrix`` on ``targets``, applied only where qubit controls[i]
+        equals control_values[i] (every control 1 when not given)."""
         m = np.asarray(matrix, dtype=np.complex128)
         d = 1 << len(targets)
         if m.shape != (d, d):
             raise ValueError("matrix side must be 2**len(targets)")
-        return Operation(m, targets, controls, label)
+        return Operation(m, targets, controls, label, control_values)
 
     # -- QuantumOperation (src/operation.rs:34-38, 325-337) ------------------
     def matrix(self) -> np.ndarray:
@@ -78,8 +84,10 @@ class Operation:
         d_all = 1 << (k + c)
         full = np.eye(d_all, dtype=np.complex128)
         d = 1 << k
-        base = d_all - d  # all control bits (the high matrix bits) set
-        full[base:, base:] = self._matrix
+        cv = self._cvals if self._cvals is not None else [True] * c
+        pattern = sum(1 << i for i, v in enumerate(cv) if v)
+        base = pattern << k  # the control bits are the high matrix bits
+        full[base:base + d, base:base + d] = self._matrix
         return full
 
     def targets(self) -> List[int]:
@@ -100,6 +108,10 @@ class Operation:
     @property
     def controls(self) -> List[int]:
         return list(self._controls)
+
+    @property
+    def control_values(self) -> Optional[List[bool]]:
+        return None if self._cvals is None else list(self._cvals)
 
     @property
     def is_structured(self) -> bool:

@@ -211,7 +211,12 @@ class Register:
         m = _c128(op.active_matrix)
         if op.is_structured:
             c = _sizes(op.controls)
-            _check(L.qs_apply_controlled(self._h, _vp(c), None, c.size, _vp(m), _vp(t), t.size))
+            cv = op.control_values
+            cvp = None
+            if cv is not None:
+                cva = np.ascontiguousarray(np.asarray(cv, dtype=np.uint8))
+                cvp = _vp(cva)
+            _check(L.qs_apply_controlled(self._h, _vp(c), cvp, c.size, _vp(m), _vp(t), t.size))
         else:
             rows, cols = (m.shape + (0, 0))[:2] if m.ndim == 2 else (m.size, 1)
             _check(L.qs_apply(self._h, _vp(m), rows, cols, _vp(t), t.size))
@@ -335,6 +340,14 @@ class Register:
 
     def get_state(self) -> np.ndarray:
         return self.state
+
+    def amplitudes(self, indices) -> np.ndarray:
+        """``state[i]`` for the given basis indices (complex128), without
+        copying the whole state to the host."""
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint64).reshape(-1))
+        out = np.empty(idx.size, dtype=np.complex128)
+        _check(_lib.load().qs_get_amplitudes(self._h, _vp(idx), idx.size, _vp(out)))
+        return out
 
     def set_state(self, value) -> "Register":
         self.state = value

@@ -1,0 +1,148 @@
+"""The measured workloads at their full sizes on the GPU.
+
+* 26 qubits, complex128 (1 GiB): the random-circuit construction of the
+  12-qubit golden case.  The first two layers are compared per amplitude
+  with the multi-core oracle (1e-12); all 20 layers, followed by their
+  inverse, must return |0...0> within 1e-12 per amplitude (a mirror circuit).
+* 30 qubits (16 GiB complex128 + 8 GiB complex64), 20 layers with 8
+  Toffoli/CCZ per layer: the complex64 run against the complex128 run
+  (infidelity <= 1e-6) and the complex128 mirror circuit back to |0...0>
+  (1e-12 per amplitude, compared on the device over all 2^30 amplitudes).
+* 32 qubits (64 GiB complex128, 32 GiB complex64): Grover with 12
+  iterations, fused (5-qubit 32 x 32 Hadamard blocks) and unfused (one gate
+  per qubit), against the closed form built on the device as its own
+  register and compared over all 2^32 amplitudes (1e-12 for complex128,
+  infidelity <= 1e-6 for complex64).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from quaru_amd import Register, circuits
+from quaru_amd import operation as ops
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _zero(n: int, dtype: str = "c128") -> Register:
+    return Register.new([False] * n, dtype)
+
+
+def test_config2_26q_prefix_matches_oracle():
+    n = 26
+    w = circuits.random_circuit(n, 2, seed=2026, measure=False)
+    print(f"26q prefix: {len(w.ops)} gates", flush=True)
+    r = _zero(n)
+    for op in w.ops:
+        r.apply(op)
+    ref = np.zeros(1 << n, dtype=np.complex128)
+    ref[0] = 1.0
+    scratch = np.empty_like(ref)
+    for op in w.ops:
+        oracle.apply_fast_omp(ref, scratch, op.matrix(), op.targets(), _threads())
+    got = r.state
+    assert float(np.max(np.abs(got - ref))) < TOL
+
+
+def test_config2_26q_mirror_returns_to_zero():
+    n = 26
+    w = circuits.random_circuit(n, 20, seed=2026, measure=False)
+    assert len(w.ops) == 26 * 20 + 25 * 20
+    r = _zero(n)
+    for op in w.ops:
+        r.apply(op)
+    assert abs(r.norm_sqr() - 1.0) < 1e-12
+    for op in circuits.inverse(w.ops):
+        r.apply(op)
+    assert r.equals(_zero(n), TOL)
+
+
+def test_config3_30q_c64_vs_c128_and_mirror():
+    n = 30
+    w = circuits.random_circuit(n, 20, seed=2025, triples=8, measure=False)
+    print(f"30q: {len(w.ops)} gates", flush=True)
+    r128 = _zero(n)
+    for op in w.ops:
+        r128.apply(op)
+    r64 = _zero(n, "c64")
+    for op in w.ops:
+        r64.apply(op)
+    inf = r64.infidelity(r128)
+    print(f"30q infidelity c64 vs c128: {inf:.3e}", flush=True)
+    assert inf <= 1e-6
+    del r64
+    for op in circuits.inverse(w.ops):
+        r128.apply(op)
+    assert r128.equals(_zero(n), TOL)
+
+
+def _grover_closed_form_register(n: int, t: int, w: int, dtype: str) -> Register:
+    """The state the closed form gives after t iterations, built on the
+    device: ao * sum_x |x> (H on every qubit, then the scalar ao * 2^(n/2)
+    as a 1-qubit multiple of I), then amplitude w scaled to aw."""
+    aw, ao = circuits.grover_closed_form(n, t, w)
+    e = Register.new([False] * n, dtype)
+    e.apply_all(ops.hadamard(0))
+    s = ao * 2.0 ** (n / 2)
+    e.apply(ops.Operation(np.eye(2, dtype=np.complex128) * s, [0]))
+    f = aw / ao
+    m = np.diag([1.0, f]) if w & 1 else np.diag([f, 1.0])
+    controls = list(range(1, n))
+    e.apply(ops.Operation.controlled(controls, m.astype(np.complex128), [0],
+                                     control_values=[bool((w >> q) & 1) for q in controls]))
+    return e
+
+
+def test_grover_closed_form_register_small():
+    # the device-built comparison state is itself the closed form (n = 10)
+    n, t = 10, 4
+    w, opl = circuits.grover_operations(n, t, seed=9, fused=False)
+    aw, ao = circuits.grover_closed_form(n, t, w)
+    e = _grover_closed_form_register(n, t, w, "c128")
+    want = np.full(1 << n, ao, dtype=np.complex128)
+    want[w] = aw
+    assert float(np.max(np.abs(e.state - want))) < 1e-14
+    r = _zero(n)
+    for op in opl:
+        r.apply(op)
+    assert r.equals(e, TOL)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_config4_32q_grover_c128(fused):
+    n, t = 32, 12
+    w, opl = circuits.grover_operations(n, t, seed=4, fused=fused)
+    print(f"32q grover c128 fused={fused}: {len(opl)} ops, winner {w}", flush=True)
+    r = _zero(n)
+    for op in opl:
+        r.apply(op)
+    aw, ao = circuits.grover_closed_form(n, t, w)
+    rng = np.random.default_rng(3)
+    idx = np.concatenate([[w, 0, (1 << n) - 1], rng.integers(0, 1 << n, 64)]).astype(np.uint64)
+    amps = r.amplitudes(idx)
+    want = np.where(idx == w, aw, ao)
+    assert float(np.max(np.abs(amps - want))) < TOL
+    e = _grover_closed_form_register(n, t, w, "c128")
+    assert r.equals(e, TOL)
+
+
+def test_config4_32q_grover_c64():
+    n, t = 32, 12
+    w, opl = circuits.grover_operations(n, t, seed=4, fused=True)
+    r = _zero(n, "c64")
+    for op in opl:
+        r.apply(op)
+    e = _grover_closed_form_register(n, t, w, "c128")
+    inf = r.infidelity(e)
+    print(f"32q grover c64 infidelity vs closed form: {inf:.3e}", flush=True)
+    assert inf <= 1e-6
+    assert abs(r.norm_sqr() - 1.0) < 1e-5
