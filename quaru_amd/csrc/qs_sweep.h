@@ -52,4 +52,15 @@ hipError_t sweep_execute(SweepEngine *e, SweepRunner &run, const std::vector<Gat
 // Register bits per lane used for a dtype (a tile is 2^(6 + bits) amplitudes).
 int sweep_reg_bits(bool c64);
 
+// Qubits 0..b-1 sit on lane bits 0..b-1 of every tile, so that each group of
+// 2^b lanes reads 2^b consecutive amplitudes (b = 4: 256-byte runs for
+// complex128, 128-byte runs for complex64); the other 6 - b lane bits take
+// qubits the planner chooses per sweep.
+int sweep_low_bits(bool c64);
+
+// Tile qubits at or above sweep_page_qubit() each double the pages a wave
+// touches; at most A + sweep_high_lanes() of them go into one sweep.
+int sweep_page_qubit(bool c64);
+int sweep_high_lanes();
+
 }  // namespace qs

@@ -1,5 +1,5 @@
 // qs_sweep.hip -- fused sweep kernel launch and its planner (see qs_sweep.h
-// and qs_sweep_device.h).
+// and qs_sweep_kernel.h).
 //
 // One sweep reads and writes the state once (2 * 2^n * s bytes, s = bytes
 // per amplitude) and applies every gate packed into it.  This is how the
@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "qs_sweep.h"
@@ -31,7 +32,6 @@ namespace {
 
 using namespace qs::sweep;
 
-constexpr int kLowBits = 4;      // qubits 0..3 are lane bits 0..3 of every tile
 constexpr int kMaxOps = 512;     // op records per sweep
 constexpr int kWindow = 384;     // look-ahead over unscheduled gates
 constexpr size_t kMaxBlobBytes = 64 * 1024;
@@ -71,13 +71,13 @@ bool is_x(const Gate &g) {
          g.m[3].re == 0 && g.m[3].im == 0;
 }
 
-// Gates the sweep kernel can apply.
-bool sweepable(const Gate &g) {
+// Gates the sweep kernel can apply (`lowb` = qubits fixed on the low lane bits).
+bool sweepable(const Gate &g, int lowb) {
   switch (g.kind) {
     case Gate::kDense:
       if (g.na == 1) return true;
       // both qubits of a 4x4 go to register slots, which low qubits never are
-      return g.na == 2 && g.act[0] >= kLowBits && g.act[1] >= kLowBits;
+      return g.na == 2 && g.act[0] >= lowb && g.act[1] >= lowb;
     case Gate::kDiag: return g.na <= 2;
     case Gate::kSparse: return true;
     default: return false;
@@ -112,13 +112,22 @@ struct Plan {
   std::vector<int> gates;
 };
 
-bool try_place(const Gate &g, int A, Plan &pl) {
+// Tile shape limits of one sweep: `lowb` fixed low qubits, A register
+// slots, and at most `hcap` tile qubits at or above the page-stride qubit.
+struct TileShape {
+  int A, lowb, hcap;
+  uint64_t high;  // qubits whose stride spans a GPU page or more
+};
+
+bool try_place(const Gate &g, const TileShape &ts, Plan &pl) {
   if (g.kind != Gate::kDense) return true;
-  const uint64_t low = (1ull << kLowBits) - 1ull;
+  const uint64_t low = (1ull << ts.lowb) - 1ull;
   uint64_t need = 0;
   for (int j = 0; j < g.na; ++j) need |= 1ull << g.act[j];
   const uint64_t other = pl.other | (need & ~low);
-  if (popc(other) > A + 2) return false;
+  if (popc(other) > ts.A + (kLaneBits - ts.lowb)) return false;
+  if (popc(other & ts.high) > ts.hcap) return false;
+  const int A = ts.A;
   uint64_t mr = pl.must_reg;
   if (g.na == 2) {
     mr |= need;
@@ -211,8 +220,17 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     }
     return hipSuccess;
   }
-  const uint64_t low = (1ull << kLowBits) - 1ull;
+  const int lowb = sweep_low_bits(run.dtype_c64);
+  const int nfree = kLaneBits - lowb;  // lane bits the planner assigns per sweep
+  const uint64_t low = (1ull << lowb) - 1ull;
   const uint64_t allq = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  // A wave reads 2^(high tile qubits) separate pages; on register slots they
+  // cost one page per load instruction, on lane bits they multiply the pages
+  // of every load instruction.  High qubits go to register slots first and at
+  // most `sweep_high_lanes` of them to lane bits.
+  const int hq = sweep_page_qubit(run.dtype_c64);
+  const TileShape ts{A, lowb, A + sweep_high_lanes(),
+                     hq >= 64 ? 0ull : (allq & ~((1ull << hq) - 1ull))};
   std::vector<uint64_t> gx(G), gz(G);
   for (size_t i = 0; i < G; ++i) gate_masks(gates[i], gx[i], gz[i]);
   std::vector<char> done(G, 0);
@@ -220,10 +238,11 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
   std::vector<OpRec> ops;
   std::vector<R> pool;
   std::vector<char> blob;
+  static const int max_gates = env_int("QS_SWEEP_MAX_GATES", 1 << 30);
   while (true) {
     while (first < G && done[first]) ++first;
     if (first == G) break;
-    if (!sweepable(gates[first])) {
+    if (!sweepable(gates[first], lowb)) {
       hipError_t e = run.single(gates[first]);
       if (e != hipSuccess) return e;
       done[first] = 1;
@@ -240,9 +259,9 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
       const Gate &g = gates[i];
       const bool movable = !(gx[i] & (bx | bz)) && !(gz[i] & bx);
       const bool room =
-          nops + op_count(g) <= kMaxOps &&
+          (int)pl.gates.size() < max_gates && nops + op_count(g) <= kMaxOps &&
           blob_bytes<R>(nops + op_count(g), npool + pool_count(g)) <= kMaxBlobBytes;
-      if (movable && room && sweepable(g) && try_place(g, A, pl)) {
+      if (movable && room && sweepable(g, lowb) && try_place(g, ts, pl)) {
         pl.gates.push_back((int)i);
         done[i] = 1;
         nops += op_count(g);
@@ -250,7 +269,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
       } else {
         bx |= gx[i];
         bz |= gz[i];
-        if (!sweepable(g)) bx |= gz[i];
+        if (!sweepable(g, lowb)) bx |= gz[i];
         if ((bx & allq) == allq) break;
       }
     }
@@ -265,15 +284,19 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
       continue;
     }
     // ---- roles: register slots for the most-used qubits (4x4 qubits
-    // first), the rest of the gate qubits on lane bits 4 and 5, then fillers
+    // first), the rest of the gate qubits on the free lane bits, then fillers
     std::vector<int> regs, lanes;
     for (int q = 0; q < n; ++q)
       if ((pl.must_reg >> q) & 1ull) regs.push_back(q);
     std::vector<int> rest;
     for (int q = 0; q < n; ++q)
       if (((pl.other & ~pl.must_reg) >> q) & 1ull) rest.push_back(q);
-    std::stable_sort(rest.begin(), rest.end(),
-                     [&](int a, int b) { return pl.cnt[a] > pl.cnt[b]; });
+    // high qubits first (register slots), then by use count
+    std::stable_sort(rest.begin(), rest.end(), [&](int a, int b) {
+      const bool ha = (ts.high >> a) & 1ull, hb = (ts.high >> b) & 1ull;
+      if (ha != hb) return ha;
+      return pl.cnt[a] > pl.cnt[b];
+    });
     for (int q : rest) {
       if ((int)regs.size() < A)
         regs.push_back(q);
@@ -281,10 +304,10 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
         lanes.push_back(q);
     }
     uint64_t P = low | pl.other;
-    for (int q = kLowBits; q < n && ((int)regs.size() < A || lanes.size() < 2); ++q) {
+    for (int q = lowb; q < n && ((int)regs.size() < A || (int)lanes.size() < nfree); ++q) {
       if ((P >> q) & 1ull) continue;
       P |= 1ull << q;
-      if (lanes.size() < 2)
+      if ((int)lanes.size() < nfree)
         lanes.push_back(q);
       else
         regs.push_back(q);
@@ -297,13 +320,13 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     }
     int slot[64];
     for (int q = 0; q < 64; ++q) slot[q] = -1;
-    for (int i = 0; i < kLowBits; ++i) {
+    for (int i = 0; i < lowb; ++i) {
       h.lg[i] = 1ull << i;
       slot[i] = (int)kLaneSlot + i;
     }
-    for (int i = 0; i < 2; ++i) {
-      h.lg[kLowBits + i] = 1ull << lanes[i];
-      slot[lanes[i]] = (int)kLaneSlot + kLowBits + i;
+    for (int i = 0; i < nfree; ++i) {
+      h.lg[lowb + i] = 1ull << lanes[i];
+      slot[lanes[i]] = (int)kLaneSlot + lowb + i;
     }
     for (int j = 0; j < A; ++j) {
       h.gb[j] = 1ull << regs[j];
@@ -343,6 +366,25 @@ struct SweepEngine {
 
 SweepEngine *sweep_engine_new() { return new SweepEngine(); }
 void sweep_engine_free(SweepEngine *e) { delete e; }
+
+// First qubit whose index stride (2^q amplitudes) spans 2 MiB, the GPU's
+// large-page size: 17 for complex128, 18 for complex64.
+int sweep_page_qubit(bool c64) {
+  return env_int(c64 ? "QS_SWEEP_PAGE_Q_C64" : "QS_SWEEP_PAGE_Q_C128", c64 ? 18 : 17);
+}
+
+// High qubits allowed on lane bits per sweep.  Measured on the 30-qubit
+// random circuit, capping them costs more in extra sweeps than it saves per
+// sweep, so the default is no cap.
+int sweep_high_lanes() { return std::max(0, env_int("QS_SWEEP_HIGH_LANES", kLaneBits)); }
+
+// Measured: with 3 fixed low qubits (128-byte runs) the planner saves ~13%
+// of the sweeps but each sweep is ~17% slower on the 30-qubit circuit, so
+// both dtypes keep 4 (256-byte runs for complex128).
+int sweep_low_bits(bool c64) {
+  const int b = env_int(c64 ? "QS_SWEEP_LOW_C64" : "QS_SWEEP_LOW_C128", 4);
+  return std::min(kLaneBits, std::max(0, b));
+}
 
 int sweep_reg_bits(bool c64) {
   const int a = env_int(c64 ? "QS_SWEEP_A_C64" : "QS_SWEEP_A_C128", 4);

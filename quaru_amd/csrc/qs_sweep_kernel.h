@@ -1,12 +1,13 @@
-// qs_sweep_device.h -- device side of the fused sweep engine: the sweep
+// qs_sweep_kernel.h -- device side of the fused sweep engine: the sweep
 // descriptor layout and the sweep kernel.  Included by qs_sweep.hip only.
 //
 // A sweep is one pass over the state.  Each wave (64 lanes) owns a tile of
 // 2^(6+A) amplitudes: every lane keeps 2^A of them in registers.  The tile
 // index bits are assigned as
-//   lane bits 0..3  <- qubits 0..3 (16 lanes read 16 consecutive amplitudes:
-//                      coalesced 256-byte (c128) / 128-byte (c64) runs),
-//   lane bits 4..5  <- two more tile qubits,
+//   lane bits 0..b-1 <- qubits 0..b-1 (b = 4 by default: 16 lanes read 16
+//                       consecutive amplitudes, coalesced 256-byte (c128) /
+//                       128-byte (c64) runs),
+//   lane bits b..5   <- more tile qubits chosen per sweep,
 //   register slots  <- A more tile qubits,
 // and the remaining index bits come from the tile id.  A gate on a register
 // qubit pairs registers inside a lane; a gate on a lane qubit fetches the
@@ -41,6 +42,10 @@ constexpr int kThreads = 256;  // 4 waves = 4 independent tiles per workgroup
 constexpr int kWave = 64;
 constexpr int kLaneBits = 6;
 constexpr int kMaxA = 5;
+#ifndef QS_XCHG_BATCH
+#define QS_XCHG_BATCH 4
+#endif
+constexpr int kXchgBatch = QS_XCHG_BATCH;  // registers exchanged per scheduling group
 
 enum : uint32_t {
   OP_U1 = 1,      // uncontrolled 2x2 on one qubit (register slot or lane bit)
@@ -241,8 +246,9 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
       a.re[r] = ur;
       a.im[r] = ui;
     }
-    // keep one register's exchange live at a time
-    __builtin_amdgcn_sched_barrier(0);
+    // exchanges of kXchgBatch registers in flight at a time: enough to hide
+    // the LDS latency, few enough to bound the extra registers
+    if ((r % kXchgBatch) == kXchgBatch - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -276,7 +282,7 @@ __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
     const bool on = pred_on<A>(p, g, r);
     a.re[r] = on ? yr : xr;
     a.im[r] = on ? yi : xi;
-    __builtin_amdgcn_sched_barrier(0);
+    if ((r % kXchgBatch) == kXchgBatch - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -404,48 +410,11 @@ __device__ __forceinline__ void dispatch_dense2(Amp<R, 1 << A> &a, const QS_CONS
   }
 }
 
-// One wave per tile: load, apply every op, store.
-template <typename R, int A, typename I>
-__global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restrict__ psi,
-                                                    const char *__restrict__ blob_g) {
-  using V = typename SV2<R>::T;
-  constexpr int T = kLaneBits + A;
-  constexpr int NA = 1 << A;
-  const QS_CONST char *blob = (const QS_CONST char *)blob_g;
-  const QS_CONST SweepHeader &h = *(const QS_CONST SweepHeader *)blob;
-  const size_t hb = (sizeof(SweepHeader) + 15) & ~(size_t)15;
-  const QS_CONST OpRec *ops = (const QS_CONST OpRec *)(blob + hb);
-  const QS_CONST R *pool =
-      (const QS_CONST R *)(blob + hb + (size_t)h.nops * sizeof(OpRec));
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / kWave) + (threadIdx.x / kWave);
-  if (tile >= h.ntiles) return;  // whole waves only
-  Regs<A, I> g;
-  g.mask = 0;
-#pragma unroll
-  for (int j = 0; j < A; ++j) {
-    g.gb[j] = (I)h.gb[j];
-    g.mask |= g.gb[j];
-  }
-  I lo = 0;
-#pragma unroll
-  for (int i = 0; i < kLaneBits; ++i) lo |= ((lane >> i) & 1) ? (I)h.lg[i] : (I)0;
-  Ctx<I> c;
-  c.gbase = (I)tile_base<T>(tile, h) | lo;
-  c.lane = lane;
-  Amp<R, NA> a;
-  {
-    typename SNT<R>::T v[NA];
-    const typename SNT<R>::T *src = reinterpret_cast<const typename SNT<R>::T *>(psi);
-#pragma unroll
-    for (int r = 0; r < NA; ++r) v[r] = __builtin_nontemporal_load(src + (c.gbase | goff<A>(g, r)));
-#pragma unroll
-    for (int r = 0; r < NA; ++r) {
-      a.re[r] = v[r].x;
-      a.im[r] = v[r].y;
-    }
-  }
-  const int nops = (int)h.nops;
+// Every op of the sweep on one wave's tile held in registers.
+template <int A, typename R, typename I>
+__device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec *ops,
+                                        const QS_CONST R *pool, int nops, const Ctx<I> &c,
+                                        const Regs<A, I> &g) {
   for (int o = 0; o < nops; ++o) {
     const QS_CONST OpRec &op = ops[o];
     const QS_CONST R *m = pool + op.moff;
@@ -459,6 +428,76 @@ __global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restri
       default: break;
     }
   }
+}
+
+// The descriptor's parts, as seen through the constant address space.
+template <typename R>
+struct Blob {
+  const QS_CONST SweepHeader *h;
+  const QS_CONST OpRec *ops;
+  const QS_CONST R *pool;
+};
+
+template <typename R>
+__device__ __forceinline__ Blob<R> open_blob(const char *blob_g) {
+  const QS_CONST char *blob = (const QS_CONST char *)blob_g;
+  Blob<R> b;
+  b.h = (const QS_CONST SweepHeader *)blob;
+  const size_t hb = (sizeof(SweepHeader) + 15) & ~(size_t)15;
+  b.ops = (const QS_CONST OpRec *)(blob + hb);
+  b.pool = (const QS_CONST R *)(blob + hb + (size_t)b.h->nops * sizeof(OpRec));
+  return b;
+}
+
+template <int A, typename I>
+__device__ __forceinline__ Regs<A, I> reg_bits(const QS_CONST SweepHeader &h) {
+  Regs<A, I> g;
+  g.mask = 0;
+#pragma unroll
+  for (int j = 0; j < A; ++j) {
+    g.gb[j] = (I)h.gb[j];
+    g.mask |= g.gb[j];
+  }
+  return g;
+}
+
+// This lane's offsets from its six lane bits.
+template <typename I>
+__device__ __forceinline__ I lane_offset(const QS_CONST SweepHeader &h, int lane) {
+  I lo = 0;
+#pragma unroll
+  for (int i = 0; i < kLaneBits; ++i) lo |= ((lane >> i) & 1) ? (I)h.lg[i] : (I)0;
+  return lo;
+}
+
+// One wave per tile: load, apply every op, store.
+template <typename R, int A, typename I>
+__global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restrict__ psi,
+                                                    const char *__restrict__ blob_g) {
+  constexpr int T = kLaneBits + A;
+  constexpr int NA = 1 << A;
+  const Blob<R> b = open_blob<R>(blob_g);
+  const QS_CONST SweepHeader &h = *b.h;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / kWave) + (threadIdx.x / kWave);
+  if (tile >= h.ntiles) return;  // whole waves only
+  const Regs<A, I> g = reg_bits<A, I>(h);
+  Ctx<I> c;
+  c.gbase = (I)tile_base<T>(tile, h) | lane_offset<I>(h, lane);
+  c.lane = lane;
+  Amp<R, NA> a;
+  {
+    typename SNT<R>::T v[NA];
+    const typename SNT<R>::T *src = reinterpret_cast<const typename SNT<R>::T *>(psi);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) v[r] = __builtin_nontemporal_load(src + (c.gbase | goff<A>(g, r)));
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      a.re[r] = v[r].x;
+      a.im[r] = v[r].y;
+    }
+  }
+  run_ops<A>(a, b.ops, b.pool, (int)h.nops, c, g);
   typename SNT<R>::T *dst = reinterpret_cast<typename SNT<R>::T *>(psi);
 #pragma unroll
   for (int r = 0; r < NA; ++r) {
