@@ -1,12 +1,12 @@
 #!/usr/bin/env python3
 """Benchmark: 30-qubit random-circuit gate application on one MI355X.
 
-Workload: n = 30 qubits, 20 layers; per layer a Haar
-2x2 on every qubit, cnot(q, q+1) for q = 0..28, then 8 three-qubit gates on
-seeded distinct triples (even -> Toffoli cnx([a,b],c), odd -> cnz([a,b],c)):
-67 gates per layer, 1340 per step.  One step applies the whole circuit
-through the GPU register's drop-in ``apply`` (fused sweep engine, the
-default) and synchronises.  The state (16 GiB complex128) stays in HBM.
+Workload: n = 30 qubits, 20 layers; per layer a Haar 2x2 on every qubit,
+cnot(q, q+1) for q = 0..28, then 8 three-qubit gates on seeded distinct
+triples (even -> Toffoli cnx([a,b],c), odd -> cnz([a,b],c)): 67 gates per
+layer, 1340 per step.  One step applies the whole circuit through the GPU
+register's drop-in ``apply`` (fused sweep engine, the default) and
+synchronises.  The state (16 GiB complex128) stays in HBM.
 
 Reported on one JSON line:
   value        gates/s (logical gate count / wall time), complex128
@@ -17,17 +17,22 @@ Reported on one JSON line:
                and its infidelity against the complex128 result
   unfused      one kernel per gate for the first layer: gates/s and the
                per-gate kernel's roofline (2 * |changed set| * s per gate)
+  adjoint      gradients/s of the 20-qubit RY/RZ + CZ-ring ansatz, batch 64,
+               complex128 and complex64, against central finite differences
+               on the same GPU
   cpu_baseline the CPU oracle (C restatement of try_apply_fast,
                src/register.rs:140-191) on a bounded sample of the same
                workload, single thread; a multi-core variant beside it
-  adjoint      (--adjoint) gradients/s of the 20-qubit ansatz, batch 64,
-               against central finite differences on the same GPU
+  grover32     (--grover32) the 32-qubit Grover workload: 5-qubit fused
+               32 x 32 blocks vs one gate per qubit, complex128 and complex64
+
+``--profile SECTION`` runs a 1 GiB copy (counter calibration) and then one
+step of that section only, for rocprofv3.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -38,6 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+CLASSES = ["sweep", "gate1", "gate2", "gatek", "big", "diag", "sparse", "other"]
 
 from quaru_amd import Register, circuits  # noqa: E402
 from quaru_amd import _lib  # noqa: E402
@@ -92,6 +98,23 @@ def run_steps(reg: Register, opl, steps: int) -> float:
         log(f"  step {s + 1}/{steps} done")
     device_sync()
     return time.perf_counter() - t0
+
+
+def timed_run(reg: Register, opl, steps: int, warmup: int):
+    """Warm up, then time `steps` passes of `opl` with the kernel timers on.
+    Returns (seconds, per-class kernel ms, per-class launches, stats)."""
+    for i in range(warmup):
+        for op in opl:
+            reg.apply(op)
+        reg.sync()
+        log(f"warmup {i + 1}/{warmup} done")
+    reg.stats(reset=True)
+    timing(reg)
+    timing_enable(reg, True)
+    dt = run_steps(reg, opl, steps)
+    timing_enable(reg, False)
+    ms, cnt = timing(reg)
+    return dt, ms, cnt, reg.stats(reset=True)
 
 
 def cpu_baseline(w: circuits.Workload, threads: int):
@@ -194,6 +217,44 @@ def adjoint_bench(n: int, layers: int, batch: int, fd_params: int, dtype: str):
     }
 
 
+def grover_bench(n: int, iterations: int, seed: int, only=None):
+    """The Grover workload at n qubits: H on all, then `iterations` x
+    [phase flip of the seeded winner, diffusion].  'fused' applies every
+    Hadamard layer as dense 32 x 32 blocks on 5 qubits (the k-qubit block
+    kernel); 'unfused' as n one-qubit gates (the sweep engine).  Each run is
+    checked against the closed form at the winner and at sampled indices."""
+    res = {"workload": f"Grover, {n} qubits, {iterations} iterations, seed {seed}"}
+    runs = [("c128_fused", "c128", True), ("c128_unfused", "c128", False),
+            ("c64_fused", "c64", True)]
+    for name, dtype, fused in runs:
+        if only and name != only:
+            continue
+        w, opl = circuits.grover_operations(n, iterations, seed=seed, fused=fused)
+        amp = 16 if dtype == "c128" else 8
+        r = Register.new([False] * n, dtype)
+        dt, ms, cnt, st = timed_run(r, opl, 1, 0)
+        aw, ao = circuits.grover_closed_form(n, iterations, w)
+        rng = np.random.default_rng(1)
+        idx = np.concatenate([[w], rng.integers(0, 1 << n, 32)]).astype(np.uint64)
+        err = float(np.max(np.abs(r.amplitudes(idx) - np.where(idx == w, aw, ao))))
+        entry = {"dtype": dtype, "ops": len(opl), "seconds": dt, "ops_per_s": len(opl) / dt,
+                 "max_abs_err_vs_closed_form": err, "sweeps": st["sweeps"],
+                 "kernel_ms": dict(zip(CLASSES, ms)), "launches": dict(zip(CLASSES, cnt))}
+        if cnt[3]:
+            avg = ms[3] / 1e3 / cnt[3]
+            byt = 2.0 * (1 << n) * amp
+            flops = (1 << n) * 32 * 8.0  # 32 complex MACs (8 real flops) per amplitude
+            entry["gatek_roofline"] = {
+                "kernel": "k_gatek<5> (dense 32 x 32 block on 5 qubits)",
+                "avg_ms": avg * 1e3, "achieved": byt / avg / 1e9, "unit": "GB/s",
+                "frac": byt / avg / HBM_PEAK, "tflops": flops / avg / 1e12}
+        res[name] = entry
+        log(f"grover {name}: {len(opl) / dt:.1f} ops/s, err {err:.2e}")
+        r.close()
+        del r
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,16 +267,32 @@ def main() -> int:
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--no-unfused", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--adjoint", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--grover32", action="store_true",
+                    help="also run the 32-qubit Grover workload (64 GiB complex128, 32 GiB "
+                         "complex64): 5-qubit fused blocks vs one gate per qubit")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: a 1 GiB torch copy (counter calibration), "
-                         "then one warmup-free step in complex128 only")
+    ap.add_argument("--profile", nargs="?", const="c128", default=None,
+                    choices=["c128", "c64", "unfused", "grover_fused", "grover_unfused",
+                             "grover_c64"],
+                    help="short run for rocprofv3: a 1 GiB torch copy (counter "
+                         "calibration), then one warmup-free step of one section")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the state-vector path is built and measured on a "
+              "single MI355X (multi-GPU qubit sharding is design only); use --gpus 1",
+              file=sys.stderr)
+        return 2
+    sections = {"c128": True, "c64": not args.no_c64, "unfused": not args.no_unfused,
+                "adjoint": not args.no_adjoint, "cpu": not args.no_cpu,
+                "grover32": args.grover32}
     if args.profile:
         args.steps, args.warmup = 1, 0
-        args.no_c64 = args.no_unfused = args.no_cpu = True
-        args.adjoint = False
+        sections = {k: False for k in sections}
+        if args.profile.startswith("grover"):
+            sections["grover32"] = True
+        else:
+            sections[args.profile] = True
         import torch
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
         dst = torch.empty_like(src)
@@ -223,113 +300,82 @@ def main() -> int:
         torch.cuda.synchronize()
         log("profile: calibration copy of 1 GiB done")
         del src, dst
-    if args.gpus != 1:
-        print("bench.py runs on one GPU: the state-vector path is built and measured on a "
-              "single MI355X (multi-GPU qubit sharding is design only); use --gpus 1",
-              file=sys.stderr)
-        return 2
 
     n = args.qubits
     w = circuits.random_circuit(n, args.layers, seed=args.seed, triples=args.triples,
                                 measure=False)
     G = len(w.ops)
-    amp = 16
     log(f"workload: {n} qubits, {args.layers} layers, {G} gates, seed {args.seed}")
-
-    # ---- complex128, fused (the drop-in default) ----
-    reg = Register.new([False] * n)
-    for i in range(args.warmup):
-        for op in w.ops:
-            reg.apply(op)
-        reg.sync()
-        log(f"warmup {i + 1}/{args.warmup} done")
-    reg.stats(reset=True)
-    timing(reg)
-    timing_enable(reg, True)
-    dt = run_steps(reg, w.ops, args.steps)
-    timing_enable(reg, False)
-    ms, cnt = timing(reg)
-    st = reg.stats(reset=True)
-    gates_per_s = G * args.steps / dt
-    sweep_avg_s = ms[0] / 1e3 / max(cnt[0], 1)
-    sweep_bytes = 2.0 * (1 << n) * amp
-    achieved = sweep_bytes / sweep_avg_s if cnt[0] else 0.0
-    total_kernel_s = sum(ms) / 1e3
-    alg_bytes = sum(2.0 * c * amp for c in w.changed) * args.steps
-    norm = reg.norm_sqr()
-    log(f"c128: {gates_per_s:.1f} gates/s, {cnt[0]} sweeps avg {sweep_avg_s * 1e3:.3f} ms "
-        f"({achieved / 1e9:.0f} GB/s), norm {norm:.15f}")
-
     out = {
         "metric": "30-qubit random-circuit gate application rate (c128, HBM-resident)",
-        "value": gates_per_s,
-        "unit": "gates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "c128",
-        "data": "synthetic",
+        "value": None, "unit": "gates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
         "config": {"workload": f"random circuit, {n} qubits, {args.layers} layers of Haar 2x2 "
                                f"on every qubit + CNOT ladder + {args.triples} Toffoli/CCZ, "
                                f"seed {args.seed}",
                    "qubits": n, "layers": args.layers, "gates_per_step": G,
-                   "state_bytes": (1 << n) * amp, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": (recorded_traffic("k_sweep_c128_30q.md")
-                                 if n == 30 and amp == 16 else None),
-                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/k_sweep_c128_30q.md",
-                     "kernel": "k_sweep (fused sweep, complex128)",
-                     "launches": cnt[0], "avg_ms": sweep_avg_s * 1e3,
-                     "bytes_per_launch": sweep_bytes},
-        "sweeps_per_step": st["sweeps"] / args.steps,
-        "gates_per_sweep": G * args.steps / max(st["sweeps"], 1),
-        "kernel_time_fraction": total_kernel_s / dt,
-        "effective_gb_per_s": alg_bytes / dt / 1e9,
-        "effective_note": "sum over gates of 2*|changed set|*16 B per wall second; fusion "
-                          "applies many gates per HBM pass, so this exceeds the HBM peak",
-        "norm_sqr": norm,
+                   "state_bytes": (1 << n) * 16, "parallelism": "single GPU"},
     }
 
+    # ---- complex128, fused (the drop-in default) ----
+    reg = None
+    if sections["c128"]:
+        amp = 16
+        reg = Register.new([False] * n)
+        dt, ms, cnt, st = timed_run(reg, w.ops, args.steps, args.warmup)
+        gates_per_s = G * args.steps / dt
+        sweep_avg_s = ms[0] / 1e3 / max(cnt[0], 1)
+        sweep_bytes = 2.0 * (1 << n) * amp
+        achieved = sweep_bytes / sweep_avg_s if cnt[0] else 0.0
+        alg_bytes = sum(2.0 * c * amp for c in w.changed) * args.steps
+        norm = reg.norm_sqr()
+        log(f"c128: {gates_per_s:.1f} gates/s, {cnt[0]} sweeps avg {sweep_avg_s * 1e3:.3f} ms "
+            f"({achieved / 1e9:.0f} GB/s), norm {norm:.15f}")
+        out["value"] = gates_per_s
+        out["ms_per_step"] = dt / args.steps * 1e3
+        out["roofline"] = {
+            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "traffic": (recorded_traffic("k_sweep_c128_30q.md") if n == 30 else None),
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/k_sweep_c128_30q.md",
+            "kernel": "k_sweep (fused sweep, complex128)", "launches": cnt[0],
+            "avg_ms": sweep_avg_s * 1e3, "bytes_per_launch": sweep_bytes}
+        out["sweeps_per_step"] = st["sweeps"] / args.steps
+        out["gates_per_sweep"] = G * args.steps / max(st["sweeps"], 1)
+        out["kernel_time_fraction"] = sum(ms) / 1e3 / dt
+        out["effective_gb_per_s"] = alg_bytes / dt / 1e9
+        out["effective_note"] = ("sum over gates of 2*|changed set|*16 B per wall second; fusion "
+                                 "applies many gates per HBM pass, so this exceeds the HBM peak")
+        out["norm_sqr"] = norm
+
     # ---- complex64 ----
-    if not args.no_c64:
+    if sections["c64"]:
         r64 = Register.new([False] * n, "c64")
-        for i in range(args.warmup):
-            for op in w.ops:
-                r64.apply(op)
-            r64.sync()
-        timing(r64)
-        timing_enable(r64, True)
-        dt64 = run_steps(r64, w.ops, args.steps)
-        timing_enable(r64, False)
-        ms64, cnt64 = timing(r64)
+        dt64, ms64, cnt64, _ = timed_run(r64, w.ops, args.steps, args.warmup)
         avg64 = ms64[0] / 1e3 / max(cnt64[0], 1)
         ach64 = 2.0 * (1 << n) * 8 / avg64 if cnt64[0] else 0.0
-        inf = r64.infidelity(reg)
-        out["c64"] = {"gates_per_s": G * args.steps / dt64,
-                      "ratio_vs_c128": (G * args.steps / dt64) / gates_per_s,
-                      "infidelity_vs_c128": inf,
-                      "roofline": {"achieved": ach64 / 1e9, "unit": "GB/s",
-                                   "frac": ach64 / HBM_PEAK, "avg_ms": avg64 * 1e3,
-                                   "launches": cnt64[0]}}
-        log(f"c64: {G * args.steps / dt64:.1f} gates/s, infidelity {inf:.3e}")
+        g64 = G * args.steps / dt64
+        entry = {"gates_per_s": g64,
+                 "roofline": {"kernel": "k_sweep (fused sweep, complex64)",
+                              "achieved": ach64 / 1e9, "unit": "GB/s",
+                              "frac": ach64 / HBM_PEAK, "avg_ms": avg64 * 1e3,
+                              "launches": cnt64[0]}}
+        if reg is not None:
+            entry["ratio_vs_c128"] = g64 / out["value"]
+            entry["infidelity_vs_c128"] = r64.infidelity(reg)
+        out["c64"] = entry
+        log(f"c64: {g64:.1f} gates/s, infidelity {entry.get('infidelity_vs_c128')}")
         r64.close()
         del r64
 
     # ---- unfused: one kernel per gate, first layer ----
-    if not args.no_unfused:
+    if sections["unfused"]:
+        amp = 16
         layer = w.ops[: n + (n - 1) + args.triples]
-        reg.set_fusion(False)
-        reg.sync()
-        timing(reg)
-        timing_enable(reg, True)
-        dtu = run_steps(reg, layer, 1)
-        timing_enable(reg, False)
-        msu, cntu = timing(reg)
+        ru = reg if reg is not None else Register.new([False] * n)
+        ru.set_fusion(False)
+        dtu, msu, cntu, _ = timed_run(ru, layer, 1, 0)
         g1 = msu[1] / 1e3 / max(cntu[1], 1)
         # dense 1q kernels in this layer are the Haar gates: 2 * 2^n * 16 B each;
         # CNOT / Toffoli (controlled) also run as 1q kernels on the controlled half.
@@ -343,20 +389,29 @@ def main() -> int:
                                        "achieved": ach_u / 1e9, "unit": "GB/s",
                                        "frac": ach_u / HBM_PEAK, "avg_ms": g1 * 1e3,
                                        "launches": cntu[1]},
-                          "kernel_ms": dict(zip(["sweep", "gate1", "gate2", "gatek", "big",
-                                                 "diag", "sparse", "other"], msu))}
+                          "kernel_ms": dict(zip(CLASSES, msu))}
         log(f"unfused: {len(layer) / dtu:.1f} gates/s, gate1 {ach_u / 1e9:.0f} GB/s")
-        reg.set_fusion(True)
-    reg.close()
-    del reg
+        ru.set_fusion(True)
+        if ru is not reg:
+            ru.close()
+    if reg is not None:
+        reg.close()
+        del reg
 
-    if args.adjoint:
+    if sections["adjoint"]:
         out["adjoint"] = adjoint_bench(20, 10, 64, 4, "c128")
         out["adjoint_c64"] = adjoint_bench(20, 10, 64, 2, "c64")
         log(f"adjoint: {out['adjoint']['grads_per_s']:.0f} grads/s, "
             f"{out['adjoint']['speedup_vs_fd']:.0f}x vs FD")
 
-    if not args.no_cpu:
+    if sections["grover32"]:
+        only = None
+        if args.profile and args.profile.startswith("grover"):
+            only = {"grover_fused": "c128_fused", "grover_unfused": "c128_unfused",
+                    "grover_c64": "c64_fused"}[args.profile]
+        out["grover32"] = grover_bench(32, 12, seed=4, only=only)
+
+    if sections["cpu"]:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_threads)
 
     print(json.dumps(out), flush=True)

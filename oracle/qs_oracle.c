@@ -24,6 +24,14 @@
  *   orc_try_measure: basis-state round trips (tests/...:53-96), the x > p0
  *       rule on hand-computed states, collapse normalisation.
  *   orc_sample_all_sequential: n calls of orc_try_measure (its definition).
+ *
+ * Parity unpinned (nothing available here can fix them):
+ *   - the uniform stream of measure: the reference draws x from the
+ *     OS-seeded rand::thread_rng (src/register.rs:317-318), so no run of it
+ *     is reproducible; the oracle and the engine take x as an argument and
+ *     share only the rule res = x > prob_0.
+ *   - bitwise rounding of ndarray's unrolled_dot for rows of length >= 8
+ *     (see unrolled_dot below): restated from its published source, not run.
  */
 #include <math.h>
 #include <stdint.h>
