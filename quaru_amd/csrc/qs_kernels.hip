@@ -63,6 +63,10 @@ __device__ __forceinline__ T czero() {
 
 constexpr int kBlock = 256;
 
+// Read-only data every thread reads at the same address (gate matrices):
+// through the constant address space the compiler keeps the reads scalar.
+#define QS_KCONST __attribute__((address_space(4)))
+
 static inline unsigned grid_for(uint64_t work, uint64_t per_block) {
   uint64_t g = (work + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -198,55 +202,204 @@ __global__ __launch_bounds__(kBlock) void k_gate2(typename V2<R>::T *__restrict_
   }
 }
 
-// K = 3..5 active targets.  A block stages kSlots amplitudes -- G = kSlots /
-// 2^K groups -- in LDS.  Slot s holds row i = s / G of group g = s % G, so
-// consecutive lanes read consecutive groups (consecutive free bits) from HBM
-// and LDS reads of a column j are bank-conflict free.  The matrix is staged
-// transposed (mt[j * D + i] = M[i][j]) so a wave's reads of one column are a
-// broadcast.  Every slot is read before any is written: in place is safe.
+// Offset of row i of a group: the K bits of i deposited at the active
+// positions (ax.pos[j] <- bit j of i).  Wave-uniform for uniform i.
+template <int K>
+__device__ __forceinline__ uint64_t row_offset(int i, const Fixed &ax) {
+  uint64_t off = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    if ((i >> j) & 1) off |= 1ull << ax.pos[j];
+  return off;
+}
+
+// K = 3..5 active targets: one thread per group of D = 2^K amplitudes.  The
+// thread loads its D amplitudes into registers, then computes and stores one
+// output row at a time, out_i = sum_j M[i][j] a_j, so the D inputs are never
+// overwritten before all rows are done (in place is safe: groups are
+// disjoint).  The matrix (row-major {re, im}, the same for every thread) is
+// read through the constant address space: its entries arrive on scalar
+// loads and enter the FMAs as scalar operands, so the only per-amplitude
+// memory traffic is the state itself.  Consecutive threads take consecutive
+// groups, i.e. consecutive free index bits: with the low qubits free, each
+// of the D loads of a wave is a run of 64 consecutive amplitudes.
 template <typename R, int K>
 __global__ __launch_bounds__(kBlock) void k_gatek(typename V2<R>::T *__restrict__ psi,
                                                   uint64_t ngroups, Fixed fx, Fixed ax,
-                                                  const typename V2<R>::T *__restrict__ mt) {
+                                                  const R *__restrict__ m_g) {
   using T = typename V2<R>::T;
   constexpr int D = 1 << K;
-  constexpr int kSlots = 1024;
-  constexpr int G = kSlots / D;
-  constexpr int kPer = kSlots / kBlock;
-  __shared__ T sa[kSlots];
-  __shared__ T sm[D * D];
-  for (int e = threadIdx.x; e < D * D; e += kBlock) sm[e] = mt[e];
-  const uint64_t g0 = (uint64_t)blockIdx.x * G;
-  uint64_t gidx[kPer];
+  const QS_KCONST R *m = (const QS_KCONST R *)m_g;
+  const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ngroups) return;
+  const uint64_t base = expand(g, fx);
+  R ar[D], ai[D];
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int s = threadIdx.x + r * kBlock;
-    const int g = s % G, i = s / G;
-    gidx[r] = ~0ull;
-    if (g0 + g < ngroups) {
-      // offset of row i: deposit the K bits of i at the active positions
-      uint64_t off = 0;
+  for (int j = 0; j < D; ++j) {
+    const T v = psi[base | row_offset<K>(j, ax)];
+    ar[j] = v.x;
+    ai[j] = v.y;
+  }
+  for (int i = 0; i < D; ++i) {
+    const QS_KCONST R *row = m + 2 * D * i;
+    // two partial sums per component: independent FMA chains
+    R sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
 #pragma unroll
-      for (int j = 0; j < K; ++j)
-        if ((i >> j) & 1) off |= 1ull << ax.pos[j];
-      gidx[r] = expand(g0 + g, fx) | off;
-      sa[s] = psi[gidx[r]];
+    for (int j = 0; j < D; j += 2) {
+      const R mr0 = row[2 * j], mi0 = row[2 * j + 1];
+      const R mr1 = row[2 * j + 2], mi1 = row[2 * j + 3];
+      sr0 = fma(mr0, ar[j], sr0);
+      sr0 = fma(-mi0, ai[j], sr0);
+      si0 = fma(mr0, ai[j], si0);
+      si0 = fma(mi0, ar[j], si0);
+      sr1 = fma(mr1, ar[j + 1], sr1);
+      sr1 = fma(-mi1, ai[j + 1], sr1);
+      si1 = fma(mr1, ai[j + 1], si1);
+      si1 = fma(mi1, ar[j + 1], si1);
     }
+    T out;
+    out.x = sr0 + sr1;
+    out.y = si0 + si1;
+    psi[base | row_offset<K>(i, ax)] = out;
+  }
+}
+
+// Bit layout of one staged block of k_gatek_staged: 64 groups x 2^K rows =
+// 2^(6+K) slots.  Slot bit b sits at state index bit pa[b] (the active
+// positions and the block's 6 lowest free positions, ascending), so the
+// lanes of a staging load vary the lowest index bits the block has.  gslot[k]
+// is the slot bit that carries bit k of the group number, jslot[k] the slot
+// bit that carries matrix bit k.
+struct StageMap {
+  uint8_t pa[16];
+  uint8_t gslot[6];
+  uint8_t jslot[8];
+};
+
+// LDS position of slot s: the low 5 bits are XORed with the next 5, which
+// spreads both the staging pattern (lanes = consecutive slots) and the
+// compute pattern (lanes = groups) over the banks.
+__device__ __forceinline__ int stage_phys(int s) { return s ^ ((s >> 5) & 31); }
+
+// K = 3..5 active targets when low index bits are fixed (active or
+// controls): the direct kernel's loads would then stride by 2^(lowest free
+// bit) amplitudes per lane.  One wave stages 64 groups through LDS with
+// loads over the block's lowest index bits, each lane computes its own group
+// from registers (matrix on scalar loads, as in k_gatek), writes its outputs
+// back to its own slots, and the wave stores the block the way it loaded it.
+template <typename R, int K>
+__global__ __launch_bounds__(64) void k_gatek_staged(typename V2<R>::T *__restrict__ psi,
+                                                     uint64_t ngroups, Fixed fx, StageMap sm,
+                                                     const R *__restrict__ m_g) {
+  using T = typename V2<R>::T;
+  constexpr int D = 1 << K;
+  constexpr int kSlots = 64 * D;
+  __shared__ T st[kSlots];
+  const QS_KCONST R *m = (const QS_KCONST R *)m_g;
+  const int lane = threadIdx.x;
+  const uint64_t g0 = (uint64_t)blockIdx.x * 64;
+  if (g0 >= ngroups) return;  // whole blocks only: ngroups is a multiple of 64
+  const uint64_t block_base = expand(g0, fx);
+  // address bits of this lane's low 6 slot bits
+  uint64_t lp = 0;
+#pragma unroll
+  for (int b = 0; b < 6; ++b)
+    if ((lane >> b) & 1) lp |= 1ull << sm.pa[b];
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    uint64_t rp = 0;  // uniform: address bits of slot bits 6.. (value r)
+#pragma unroll
+    for (int b = 0; b < K; ++b)
+      if ((r >> b) & 1) rp |= 1ull << sm.pa[6 + b];
+    st[stage_phys(r * 64 + lane)] = psi[block_base | lp | rp];
+  }
+  __syncthreads();
+  int tp = 0;  // this lane's group number on the slot bits
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    if ((lane >> k) & 1) tp |= 1 << sm.gslot[k];
+  R ar[D], ai[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    int jp = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if ((j >> k) & 1) jp |= 1 << sm.jslot[k];
+    const T v = st[stage_phys(tp | jp)];
+    ar[j] = v.x;
+    ai[j] = v.y;
+  }
+  for (int i = 0; i < D; ++i) {
+    const QS_KCONST R *row = m + 2 * D * i;
+    R sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
+#pragma unroll
+    for (int j = 0; j < D; j += 2) {
+      const R mr0 = row[2 * j], mi0 = row[2 * j + 1];
+      const R mr1 = row[2 * j + 2], mi1 = row[2 * j + 3];
+      sr0 = fma(mr0, ar[j], sr0);
+      sr0 = fma(-mi0, ai[j], sr0);
+      si0 = fma(mr0, ai[j], si0);
+      si0 = fma(mi0, ar[j], si0);
+      sr1 = fma(mr1, ar[j + 1], sr1);
+      sr1 = fma(-mi1, ai[j + 1], sr1);
+      si1 = fma(mr1, ai[j + 1], si1);
+      si1 = fma(mi1, ar[j + 1], si1);
+    }
+    int ip = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if ((i >> k) & 1) ip |= 1 << sm.jslot[k];
+    T out;
+    out.x = sr0 + sr1;
+    out.y = si0 + si1;
+    st[stage_phys(tp | ip)] = out;  // own slot: already read into registers
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int s = threadIdx.x + r * kBlock;
-    const int g = s % G, i = s / G;
-    if (gidx[r] == ~0ull) continue;
-    T acc = czero<T>();
-#pragma unroll 8
-    for (int j = 0; j < D; ++j) {
-      const T mv = sm[j * D + i];
-      acc = cmla(acc, mv.x, mv.y, sa[j * G + g]);
-    }
-    psi[gidx[r]] = acc;
+  for (int r = 0; r < D; ++r) {
+    uint64_t rp = 0;
+#pragma unroll
+    for (int b = 0; b < K; ++b)
+      if ((r >> b) & 1) rp |= 1ull << sm.pa[6 + b];
+    psi[block_base | lp | rp] = st[stage_phys(r * 64 + lane)];
   }
+}
+
+static uint64_t fx_mask(const Fixed &fx) {
+  uint64_t m = 0;
+  for (int i = 0; i < fx.nf; ++i) m |= 1ull << fx.pos[i];
+  return m;
+}
+
+// The staged layout for a gate: active positions `ax` (matrix-bit order) and
+// the lowest 6 free positions of `fx`.  Returns false when fewer than 6 free
+// positions exist (tiny registers use the direct kernel).
+static bool stage_map(const Fixed &fx, const Fixed &ax, int n, StageMap &sm) {
+  uint64_t fixed = 0;
+  for (int i = 0; i < fx.nf; ++i) fixed |= 1ull << fx.pos[i];
+  uint64_t act = 0;
+  for (int j = 0; j < ax.nf; ++j) act |= 1ull << ax.pos[j];
+  uint64_t free6 = 0;
+  int nfree = 0;
+  for (int q = 0; q < n && nfree < 6; ++q)
+    if (!((fixed >> q) & 1ull)) {
+      free6 |= 1ull << q;
+      ++nfree;
+    }
+  if (nfree < 6) return false;
+  const uint64_t P = act | free6;
+  int b = 0, gk = 0;
+  uint8_t slot_of[64];
+  for (int q = 0; q < 64; ++q)
+    if ((P >> q) & 1ull) {
+      slot_of[q] = (uint8_t)b;
+      sm.pa[b++] = (uint8_t)q;
+      if ((free6 >> q) & 1ull) sm.gslot[gk++] = (uint8_t)(b - 1);
+    }
+  // pa[] must list the slot bits low to high: slots 0..5 are the lowest six
+  // positions of P and feed the lanes; the rest follow in order.
+  for (int j = 0; j < ax.nf; ++j) sm.jslot[j] = slot_of[ax.pos[j]];
+  return true;
 }
 
 // Any K: out-of-place from `src` (a copy of the state) into psi, one thread
@@ -327,14 +480,30 @@ hipError_t launch_gate_dense(void *psi_v, int n, const Gate &g, void *d_mat, voi
                        to_matarg<R, 4>(g));
   } else if (g.na >= 3 && g.na <= 5) {
     const Fixed ax = active_positions(g);
-    const T *mt = (const T *)d_mat;
-    const unsigned grid = grid_for(nfreeN, 1024 >> g.na);
+    const R *m = (const R *)d_mat;  // row-major {re, im}
+    StageMap sm{};
+    // fixed low bits (active or control on qubits 0..3) break the direct
+    // kernel's 64-amplitude runs: stage through LDS instead
+    if ((fx_mask(fx) & 0xFull) && stage_map(fx, ax, n, sm)) {
+      const unsigned sgrid = (unsigned)(nfreeN / 64);
+      if (g.na == 3)
+        hipLaunchKernelGGL((k_gatek_staged<R, 3>), dim3(sgrid), dim3(64), 0, s, psi, nfreeN, fx,
+                           sm, m);
+      else if (g.na == 4)
+        hipLaunchKernelGGL((k_gatek_staged<R, 4>), dim3(sgrid), dim3(64), 0, s, psi, nfreeN, fx,
+                           sm, m);
+      else
+        hipLaunchKernelGGL((k_gatek_staged<R, 5>), dim3(sgrid), dim3(64), 0, s, psi, nfreeN, fx,
+                           sm, m);
+      return hipGetLastError();
+    }
+    const unsigned grid = grid_for(nfreeN, kBlock);
     if (g.na == 3)
-      hipLaunchKernelGGL((k_gatek<R, 3>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, mt);
+      hipLaunchKernelGGL((k_gatek<R, 3>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, m);
     else if (g.na == 4)
-      hipLaunchKernelGGL((k_gatek<R, 4>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, mt);
+      hipLaunchKernelGGL((k_gatek<R, 4>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, m);
     else
-      hipLaunchKernelGGL((k_gatek<R, 5>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, mt);
+      hipLaunchKernelGGL((k_gatek<R, 5>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, m);
   } else {
     const Fixed ax = active_positions(g);
     hipError_t e = hipMemcpyAsync(scratch, psi, sizeof(T) << n, hipMemcpyDeviceToDevice, s);

@@ -360,14 +360,12 @@ hipError_t launch_one(qs_register *r, const Gate &g) {
       void *tmp = nullptr;
       hipError_t e = hipSuccess;
       if (g.na >= 3) {
+        // row-major {re, im} in the register's precision
         std::vector<T> buf(2 * d * d);
-        const bool transposed = g.na <= 5;
-        for (size_t i = 0; i < d; ++i)
-          for (size_t j = 0; j < d; ++j) {
-            const size_t dst = transposed ? j * d + i : i * d + j;
-            buf[2 * dst] = (T)g.m[i * d + j].re;
-            buf[2 * dst + 1] = (T)g.m[i * d + j].im;
-          }
+        for (size_t e = 0; e < d * d; ++e) {
+          buf[2 * e] = (T)g.m[e].re;
+          buf[2 * e + 1] = (T)g.m[e].im;
+        }
         const size_t bytes = buf.size() * sizeof(T);
         e = bytes + 256 <= r->ring_cap ? ring_upload(r, buf.data(), bytes, &dm)
                                        : big_upload(r, buf.data(), bytes, &tmp);

@@ -139,6 +139,24 @@ bool try_place(const Gate &g, const TileShape &ts, Plan &pl) {
   return true;
 }
 
+// try_place for a tile whose non-low qubits are fixed to the set S.
+bool fits_set(const Gate &g, uint64_t S, const TileShape &ts, Plan &pl) {
+  if (g.kind != Gate::kDense) return true;
+  const uint64_t low = (1ull << ts.lowb) - 1ull;
+  uint64_t need = 0;
+  for (int j = 0; j < g.na; ++j) need |= 1ull << g.act[j];
+  if (need & ~low & ~S) return false;
+  uint64_t mr = pl.must_reg;
+  if (g.na == 2) {
+    mr |= need;
+    if (popc(mr) > ts.A) return false;
+  }
+  pl.other |= need & ~low;
+  pl.must_reg = mr;
+  for (int j = 0; j < g.na; ++j) ++pl.cnt[g.act[j]];
+  return true;
+}
+
 template <typename R>
 void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::vector<R> &pool) {
   auto put = [&](const cd &v) {
@@ -152,6 +170,7 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
       op.cmask = g.smask;
       op.cval = g.spat[e];
       op.moff = (uint32_t)pool.size();
+      op.code = op_code(op.type, 0, 0);
       put(g.m[e]);
       ops.push_back(op);
     }
@@ -167,6 +186,7 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
     op.na = (uint32_t)g.na;
     for (int j = 0; j < g.na; ++j) op.act[j] = (uint32_t)g.act[j];
     for (const cd &v : g.m) put(v);
+    op.code = op_code(op.type, 0, 0);
     ops.push_back(op);
     return;
   }
@@ -180,7 +200,23 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
   }
   if (op.type != OP_X1)
     for (const cd &v : g.m) put(v);
+  op.code = op_code(op.type, op.s0, op.s1);
   ops.push_back(op);
+}
+
+// Compute units of the current device (cached per device).
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
 }
 
 template <typename R, int A>
@@ -188,6 +224,20 @@ hipError_t launch_sweep(SweepRunner &run, const void *d_blob, uint64_t ntiles) {
   using V = typename SV2<R>::T;
   const uint64_t per_block = kThreads / kWave;
   const uint64_t grid = (ntiles + per_block - 1) / per_block;
+  if constexpr (std::is_same<R, double>::value && A == 4) {
+    static const int pipe = env_int("QS_SWEEP_PIPE", 0);
+    if (pipe > 0) {
+      // persistent waves: `pipe` workgroups per compute unit
+      const uint64_t pgrid = std::min<uint64_t>(grid, (uint64_t)device_cus() * (uint64_t)pipe);
+      if (run.n <= 32)
+        hipLaunchKernelGGL((k_sweep_pipe<A, uint32_t>), dim3((unsigned)pgrid), dim3(kThreads), 0,
+                           run.stream, (double2 *)run.psi, (const char *)d_blob);
+      else
+        hipLaunchKernelGGL((k_sweep_pipe<A, uint64_t>), dim3((unsigned)pgrid), dim3(kThreads), 0,
+                           run.stream, (double2 *)run.psi, (const char *)d_blob);
+      return hipGetLastError();
+    }
+  }
   if (run.n <= 32)
     hipLaunchKernelGGL((k_sweep<R, A, uint32_t>), dim3((unsigned)grid), dim3(kThreads), 0,
                        run.stream, (V *)run.psi, (const char *)d_blob);
@@ -239,6 +289,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
   std::vector<R> pool;
   std::vector<char> blob;
   static const int max_gates = env_int("QS_SWEEP_MAX_GATES", 1 << 30);
+  static const int search_iters = std::max(0, env_int("QS_SWEEP_SEARCH", 8));
   while (true) {
     while (first < G && done[first]) ++first;
     if (first == G) break;
@@ -248,31 +299,79 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
       done[first] = 1;
       continue;
     }
-    // ---- collect the gates of one sweep
+    // ---- collect the gates of one sweep.  fixed = false: the tile's
+    // non-low qubits are taken first come, first served; fixed = true: they
+    // are the set S.
+    auto collect = [&](bool fixed, uint64_t S, Plan &pl) {
+      pl = Plan();
+      int nops = 0, npool = 0;
+      uint64_t bx = 0, bz = 0;
+      int scanned = 0;
+      for (size_t i = first; i < G && scanned < kWindow; ++i) {
+        if (done[i]) continue;
+        ++scanned;
+        const Gate &g = gates[i];
+        const bool movable = !(gx[i] & (bx | bz)) && !(gz[i] & bx);
+        const bool room =
+            (int)pl.gates.size() < max_gates && nops + op_count(g) <= kMaxOps &&
+            blob_bytes<R>(nops + op_count(g), npool + pool_count(g)) <= kMaxBlobBytes;
+        bool ok = movable && room && sweepable(g, lowb);
+        if (ok) ok = fixed ? fits_set(g, S, ts, pl) : try_place(g, ts, pl);
+        if (ok) {
+          pl.gates.push_back((int)i);
+          nops += op_count(g);
+          npool += pool_count(g);
+        } else {
+          bx |= gx[i];
+          bz |= gz[i];
+          if (!sweepable(g, lowb)) bx |= gz[i];
+          if ((bx & allq) == allq) break;
+        }
+      }
+    };
     Plan pl;
-    int nops = 0, npool = 0;
-    uint64_t bx = 0, bz = 0;
-    int scanned = 0;
-    for (size_t i = first; i < G && scanned < kWindow; ++i) {
-      if (done[i]) continue;
-      ++scanned;
-      const Gate &g = gates[i];
-      const bool movable = !(gx[i] & (bx | bz)) && !(gz[i] & bx);
-      const bool room =
-          (int)pl.gates.size() < max_gates && nops + op_count(g) <= kMaxOps &&
-          blob_bytes<R>(nops + op_count(g), npool + pool_count(g)) <= kMaxBlobBytes;
-      if (movable && room && sweepable(g, lowb) && try_place(g, ts, pl)) {
-        pl.gates.push_back((int)i);
-        done[i] = 1;
-        nops += op_count(g);
-        npool += pool_count(g);
-      } else {
-        bx |= gx[i];
-        bz |= gz[i];
-        if (!sweepable(g, lowb)) bx |= gz[i];
-        if ((bx & allq) == allq) break;
+    collect(false, 0, pl);
+    // ---- local search over the tile's qubit set: add a qubit while there
+    // is room, else swap one in for one out; keep a change when it schedules
+    // more gates.  First come, first served fills the tile with the qubits of
+    // the earliest gates, which is often not the set that reaches furthest
+    // into the circuit.
+    {
+      const int cap = A + nfree;
+      uint64_t S = pl.other;
+      Plan cand;
+      for (int it = 0; it < search_iters; ++it) {
+        bool improved = false;
+        for (int qi = lowb; qi < n; ++qi) {
+          if ((S >> qi) & 1ull) continue;
+          if (popc(S) < cap) {
+            const uint64_t S2 = S | (1ull << qi);
+            if (popc(S2 & ts.high) > ts.hcap) continue;
+            collect(true, S2, cand);
+            if (cand.gates.size() > pl.gates.size()) {
+              std::swap(pl, cand);
+              S = S2;
+              improved = true;
+            }
+            continue;
+          }
+          for (int qo = lowb; qo < n; ++qo) {
+            if (!((S >> qo) & 1ull)) continue;
+            const uint64_t S2 = (S & ~(1ull << qo)) | (1ull << qi);
+            if (popc(S2 & ts.high) > ts.hcap) continue;
+            collect(true, S2, cand);
+            if (cand.gates.size() > pl.gates.size()) {
+              std::swap(pl, cand);
+              S = S2;
+              improved = true;
+              break;  // S changed: continue with the next qubit to bring in
+            }
+          }
+        }
+        if (!improved) break;
       }
     }
+    for (int gi : pl.gates) done[gi] = 1;
     // ---- sweep or single kernels?
     double single_cost = 0.0;
     for (int gi : pl.gates) single_cost += single_amps(gates[gi], n);

@@ -59,12 +59,38 @@ enum : uint32_t {
 // Slot encoding in s0/s1: 0..A-1 register slot, kLaneSlot + k lane bit k.
 constexpr uint32_t kLaneSlot = 8;
 
+// The kernel's dispatch code: type and register slots in one number.
+// U1/C1/X1: base + slot (slot kMaxA = any lane bit); DENSE2: base + s0 *
+// kMaxA + s1.
+enum : uint32_t {
+  CODE_NONE = 0,
+  CODE_U1 = 1,                    // 1 .. 6
+  CODE_C1 = CODE_U1 + kMaxA + 1,  // 7 .. 12
+  CODE_X1 = CODE_C1 + kMaxA + 1,  // 13 .. 18
+  CODE_SPARSE = CODE_X1 + kMaxA + 1,
+  CODE_DIAG = CODE_SPARSE + 1,
+  CODE_DENSE2 = CODE_DIAG + 1     // + s0 * kMaxA + s1
+};
+
+__host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t s1) {
+  const uint32_t j0 = s0 >= kLaneSlot ? (uint32_t)kMaxA : s0;
+  switch (type) {
+    case OP_U1: return CODE_U1 + j0;
+    case OP_C1: return CODE_C1 + j0;
+    case OP_X1: return CODE_X1 + j0;
+    case OP_SPARSE: return CODE_SPARSE;
+    case OP_DIAG: return CODE_DIAG;
+    case OP_DENSE2: return CODE_DENSE2 + s0 * kMaxA + s1;  // both register slots
+    default: return CODE_NONE;
+  }
+}
+
 // 64-byte op record; complex values live in a pool of R scalars
 // ({re, im} interleaved, row-major) at pool[moff].
 struct OpRec {
   uint32_t type, s0, s1, na;
   uint32_t act[4];
-  uint32_t moff, pad0;
+  uint32_t moff, code;
   uint64_t cmask, cval;
   uint32_t pad1[2];
 };
@@ -353,81 +379,67 @@ __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec 
     cfma(sr, si, vr, vi, xr, xi);
     a.re[r] = on ? sr : xr;
     a.im[r] = on ? si : xi;
-  }
-}
-
-template <bool kCtl, int A, typename R, typename I>
-__device__ __forceinline__ void dispatch_1q(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                            const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
-  switch (op.s0) {
-    case 0: op_1q_reg<0, kCtl, A>(a, op, m, c, g); break;
-    case 1: if constexpr (A > 1) op_1q_reg<1, kCtl, A>(a, op, m, c, g); break;
-    case 2: if constexpr (A > 2) op_1q_reg<2, kCtl, A>(a, op, m, c, g); break;
-    case 3: if constexpr (A > 3) op_1q_reg<3, kCtl, A>(a, op, m, c, g); break;
-    case 4: if constexpr (A > 4) op_1q_reg<4, kCtl, A>(a, op, m, c, g); break;
-    default: op_1q_lane<kCtl, A>(a, op, m, c, g); break;
-  }
-}
-
-template <int A, typename R, typename I>
-__device__ __forceinline__ void dispatch_x1(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                            const Ctx<I> &c, const Regs<A, I> &g) {
-  switch (op.s0) {
-    case 0: op_x1_reg<0, A, R, I>(a, op, c, g); break;
-    case 1: if constexpr (A > 1) op_x1_reg<1, A, R, I>(a, op, c, g); break;
-    case 2: if constexpr (A > 2) op_x1_reg<2, A, R, I>(a, op, c, g); break;
-    case 3: if constexpr (A > 3) op_x1_reg<3, A, R, I>(a, op, c, g); break;
-    case 4: if constexpr (A > 4) op_x1_reg<4, A, R, I>(a, op, c, g); break;
-    default: op_x1_lane<A, R, I>(a, op, c, g); break;
-  }
-}
-
-template <int J0, int A, typename R, typename I>
-__device__ __forceinline__ void dispatch_dense2_b(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                                  const QS_CONST R *m, const Ctx<I> &c,
-                                                  const Regs<A, I> &g) {
-  switch (op.s1) {
-    case 0: if constexpr (J0 != 0) op_dense2<J0, 0, A>(a, op, m, c, g); break;
-    case 1: if constexpr (A > 1 && J0 != 1) op_dense2<J0, 1, A>(a, op, m, c, g); break;
-    case 2: if constexpr (A > 2 && J0 != 2) op_dense2<J0, 2, A>(a, op, m, c, g); break;
-    case 3: if constexpr (A > 3 && J0 != 3) op_dense2<J0, 3, A>(a, op, m, c, g); break;
-    case 4: if constexpr (A > 4 && J0 != 4) op_dense2<J0, 4, A>(a, op, m, c, g); break;
-    default: break;
-  }
-}
-
-template <int A, typename R, typename I>
-__device__ __forceinline__ void dispatch_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
-                                                const QS_CONST R *m, const Ctx<I> &c,
-                                                const Regs<A, I> &g) {
-  switch (op.s0) {
-    case 0: dispatch_dense2_b<0, A>(a, op, m, c, g); break;
-    case 1: if constexpr (A > 1) dispatch_dense2_b<1, A>(a, op, m, c, g); break;
-    case 2: if constexpr (A > 2) dispatch_dense2_b<2, A>(a, op, m, c, g); break;
-    case 3: if constexpr (A > 3) dispatch_dense2_b<3, A>(a, op, m, c, g); break;
-    case 4: if constexpr (A > 4) dispatch_dense2_b<4, A>(a, op, m, c, g); break;
-    default: break;
+    // without this the scheduler selects the factors of every register
+    // ahead of the multiplies, holding 2^A extra complex values live
+    if (r & 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
 // Every op of the sweep on one wave's tile held in registers.
+//
+// One flat switch over the op's code (type and register slots folded
+// together by the planner, see op_code) rather than nested switches on type
+// and slot: each nesting level is a join where the compiler copied the whole
+// amplitude array between register sets, three whole-array copies per op.
 template <int A, typename R, typename I>
 __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec *ops,
                                         const QS_CONST R *pool, int nops, const Ctx<I> &c,
                                         const Regs<A, I> &g) {
+#define QS_REG1(BASE, J, CALL) \
+  case (BASE) + (J):           \
+    if constexpr ((J) < A) CALL; \
+    break;
+#define QS_REG2(J0, J1)                                                      \
+  case CODE_DENSE2 + (J0) * kMaxA + (J1):                                    \
+    if constexpr ((J0) < A && (J1) < A && (J0) != (J1))                      \
+      op_dense2<(J0), (J1), A>(a, op, m, c, g);                              \
+    break;
+#define QS_REG2_ROW(J0) QS_REG2(J0, 0) QS_REG2(J0, 1) QS_REG2(J0, 2) QS_REG2(J0, 3) QS_REG2(J0, 4)
   for (int o = 0; o < nops; ++o) {
     const QS_CONST OpRec &op = ops[o];
     const QS_CONST R *m = pool + op.moff;
-    switch (op.type) {
-      case OP_U1: dispatch_1q<false, A>(a, op, m, c, g); break;
-      case OP_C1: dispatch_1q<true, A>(a, op, m, c, g); break;
-      case OP_X1: dispatch_x1<A>(a, op, c, g); break;
-      case OP_DENSE2: dispatch_dense2<A>(a, op, m, c, g); break;
-      case OP_SPARSE: op_sparse<A>(a, op, m, c, g); break;
-      case OP_DIAG: op_diag<A>(a, op, m, c, g); break;
+    switch (op.code) {
+      QS_REG1(CODE_U1, 0, (op_1q_reg<0, false, A>(a, op, m, c, g)))
+      QS_REG1(CODE_U1, 1, (op_1q_reg<1, false, A>(a, op, m, c, g)))
+      QS_REG1(CODE_U1, 2, (op_1q_reg<2, false, A>(a, op, m, c, g)))
+      QS_REG1(CODE_U1, 3, (op_1q_reg<3, false, A>(a, op, m, c, g)))
+      QS_REG1(CODE_U1, 4, (op_1q_reg<4, false, A>(a, op, m, c, g)))
+      case CODE_U1 + kMaxA: op_1q_lane<false, A>(a, op, m, c, g); break;
+      QS_REG1(CODE_C1, 0, (op_1q_reg<0, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_C1, 1, (op_1q_reg<1, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_C1, 2, (op_1q_reg<2, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_C1, 3, (op_1q_reg<3, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_C1, 4, (op_1q_reg<4, true, A>(a, op, m, c, g)))
+      case CODE_C1 + kMaxA: op_1q_lane<true, A>(a, op, m, c, g); break;
+      QS_REG1(CODE_X1, 0, (op_x1_reg<0, A, R, I>(a, op, c, g)))
+      QS_REG1(CODE_X1, 1, (op_x1_reg<1, A, R, I>(a, op, c, g)))
+      QS_REG1(CODE_X1, 2, (op_x1_reg<2, A, R, I>(a, op, c, g)))
+      QS_REG1(CODE_X1, 3, (op_x1_reg<3, A, R, I>(a, op, c, g)))
+      QS_REG1(CODE_X1, 4, (op_x1_reg<4, A, R, I>(a, op, c, g)))
+      case CODE_X1 + kMaxA: op_x1_lane<A, R, I>(a, op, c, g); break;
+      case CODE_SPARSE: op_sparse<A>(a, op, m, c, g); break;
+      case CODE_DIAG: op_diag<A>(a, op, m, c, g); break;
+      QS_REG2_ROW(0)
+      QS_REG2_ROW(1)
+      QS_REG2_ROW(2)
+      QS_REG2_ROW(3)
+      QS_REG2_ROW(4)
       default: break;
     }
   }
+#undef QS_REG2_ROW
+#undef QS_REG2
+#undef QS_REG1
 }
 
 // The descriptor's parts, as seen through the constant address space.
@@ -472,7 +484,11 @@ __device__ __forceinline__ I lane_offset(const QS_CONST SweepHeader &h, int lane
 
 // One wave per tile: load, apply every op, store.
 template <typename R, int A, typename I>
-__global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restrict__ psi,
+#ifndef QS_SWEEP_WAVES
+#define QS_SWEEP_WAVES 1
+#endif
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(QS_SWEEP_WAVES))) void k_sweep(typename SV2<R>::T *__restrict__ psi,
                                                     const char *__restrict__ blob_g) {
   constexpr int T = kLaneBits + A;
   constexpr int NA = 1 << A;
@@ -505,6 +521,104 @@ __global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restri
     v.x = a.re[r];
     v.y = a.im[r];
     __builtin_nontemporal_store(v, dst + (c.gbase | goff<A>(g, r)));
+  }
+}
+
+// Software-pipelined sweep for complex128: persistent waves walk tiles
+// wave_id, wave_id + W, ... (W = waves in the grid) with two LDS buffers
+// per wave.  While the ops of tile t run in registers, tile t + W is already
+// on its way into the IN buffer (global_load_lds_dwordx4: lane l's 16 bytes
+// land at buffer + 16 l), and the results of tile t - W drain from the OUT
+// buffer to HBM.  Per tile:
+//   ops(t) in registers; registers -> OUT; IN -> registers (tile t + W);
+//   DMA tile t + 2W -> IN; OUT -> HBM (tile t).
+// The wait for the IN data comes one whole tile of arithmetic after its DMA
+// and after the previous tile's stores, so neither load nor store latency
+// sits on the wave's critical path.  Every lane reads back only the LDS slots
+// its own loads and writes filled, so no barrier is needed.
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <int A, typename I>
+__device__ __forceinline__ void dma_tile(const double2 *psi, I base, const Regs<A, I> &g,
+                                         double2 *buf) {
+#pragma unroll
+  for (int r = 0; r < (1 << A); ++r)
+    __builtin_amdgcn_global_load_lds((const void *)(psi + (base | goff<A>(g, r))),
+                                     (lds_void_t *)(buf + r * kWave), 16, 0, 0);
+}
+
+template <int A, typename I>
+__global__ __launch_bounds__(kThreads) void k_sweep_pipe(double2 *__restrict__ psi,
+                                                         const char *__restrict__ blob_g) {
+  constexpr int T = kLaneBits + A;
+  constexpr int NA = 1 << A;
+  constexpr int W = kThreads / kWave;
+  __shared__ double2 lin[W][NA * kWave];
+  __shared__ double2 lout[W][NA * kWave];
+  const Blob<double> b = open_blob<double>(blob_g);
+  const QS_CONST SweepHeader &h = *b.h;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  double2 *in = lin[wv];
+  double2 *out = lout[wv];
+  const uint64_t stride = (uint64_t)gridDim.x * W;
+  uint64_t tile = (uint64_t)blockIdx.x * W + wv;
+  if (tile >= h.ntiles) return;  // whole waves only
+  const Regs<A, I> g = reg_bits<A, I>(h);
+  const I lo = lane_offset<I>(h, lane);
+  Ctx<I> c;
+  c.lane = lane;
+  c.gbase = (I)tile_base<T>(tile, h) | lo;
+  Amp<double, NA> a;
+  dma_tile<A>(psi, c.gbase, g, in);
+#pragma unroll
+  for (int r = 0; r < NA; ++r) {
+    const double2 v = in[r * kWave + lane];
+    a.re[r] = v.x;
+    a.im[r] = v.y;
+  }
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): IN read before its refill
+  uint64_t next = tile + stride;
+  I next_base = 0;
+  if (next < h.ntiles) {
+    next_base = (I)tile_base<T>(next, h) | lo;
+    dma_tile<A>(psi, next_base, g, in);
+  }
+  for (;;) {
+    run_ops<A>(a, b.ops, b.pool, (int)h.nops, c, g);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      double2 v;
+      v.x = a.re[r];
+      v.y = a.im[r];
+      out[r * kWave + lane] = v;
+    }
+    const bool more = next < h.ntiles;
+    uint64_t after = next + stride;
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const double2 v = in[r * kWave + lane];
+        a.re[r] = v.x;
+        a.im[r] = v.y;
+      }
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+      if (after < h.ntiles)
+        dma_tile<A>(psi, (I)tile_base<T>(after, h) | lo, g, in);
+    }
+    typename SNT<double>::T *dst = reinterpret_cast<typename SNT<double>::T *>(psi);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      const double2 v = out[r * kWave + lane];
+      typename SNT<double>::T w;
+      w.x = v.x;
+      w.y = v.y;
+      __builtin_nontemporal_store(w, dst + (c.gbase | goff<A>(g, r)));
+    }
+    if (!more) break;
+    c.gbase = next_base;
+    next = after;
+    if (next < h.ntiles) next_base = (I)tile_base<T>(next, h) | lo;
   }
 }
 

@@ -122,6 +122,33 @@ def gate_zoo(rng, n):
     return zoo
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_block_gates_low_and_high_targets(dtype):
+    """Dense 3..5-qubit blocks on low qubits (the LDS-staged kernel), on high
+    qubits (the direct kernel) and under controls on low qubits."""
+    rng = np.random.default_rng(11)
+    n = 14
+    opl = []
+    for k in (3, 4, 5):
+        opl.append(ops.Operation(random_unitary(rng, 1 << k), list(range(k))))
+        opl.append(ops.Operation(random_unitary(rng, 1 << k), list(range(n - k, n))[::-1]))
+        low = [int(x) for x in rng.permutation(4)[:2]]
+        rest = [int(x) for x in rng.permutation(np.arange(4, n))[:k - 1]]
+        opl.append(ops.Operation(random_unitary(rng, 1 << k), [low[0]] + rest))
+        opl.append(ops.to_controlled(ops.Operation(random_unitary(rng, 1 << (k - 1)), rest[: k - 1]),
+                                     low[1]))
+    psi0 = random_state(rng, n)
+    ref = oracle_run(psi0.copy(), opl)
+    r = Register.new([False] * n, dtype)
+    r.state = psi0
+    for op in opl:
+        r.apply(op)
+    if dtype == "c128":
+        assert maxdiff(r.state, ref) < TOL
+    else:
+        assert infid(r.state, ref) < 1e-6
+
+
 @pytest.mark.parametrize("fusion", [True, False])
 @pytest.mark.parametrize("n", [8, 12, 13])
 def test_gate_zoo_parity_c128(n, fusion):
