@@ -290,6 +290,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
   std::vector<char> blob;
   static const int max_gates = env_int("QS_SWEEP_MAX_GATES", 1 << 30);
   static const int search_iters = std::max(0, env_int("QS_SWEEP_SEARCH", 8));
+  static const int search_restarts = std::max(0, env_int("QS_SWEEP_RESTARTS", 2));
   while (true) {
     while (first < G && done[first]) ++first;
     if (first == G) break;
@@ -335,40 +336,62 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     // is room, else swap one in for one out; keep a change when it schedules
     // more gates.  First come, first served fills the tile with the qubits of
     // the earliest gates, which is often not the set that reaches furthest
-    // into the circuit.
-    {
+    // into the circuit.  The search runs from that set and from a few
+    // pseudo-random full sets (seeded by the sweep number, so plans are
+    // reproducible) and keeps the best result.
+    if (search_iters > 0) {
       const int cap = A + nfree;
-      uint64_t S = pl.other;
-      Plan cand;
-      for (int it = 0; it < search_iters; ++it) {
-        bool improved = false;
-        for (int qi = lowb; qi < n; ++qi) {
-          if ((S >> qi) & 1ull) continue;
-          if (popc(S) < cap) {
-            const uint64_t S2 = S | (1ull << qi);
-            if (popc(S2 & ts.high) > ts.hcap) continue;
-            collect(true, S2, cand);
-            if (cand.gates.size() > pl.gates.size()) {
-              std::swap(pl, cand);
-              S = S2;
-              improved = true;
+      Plan cur, cand;
+      auto climb = [&](uint64_t S, Plan &best) {
+        for (int it = 0; it < search_iters; ++it) {
+          bool improved = false;
+          for (int qi = lowb; qi < n; ++qi) {
+            if ((S >> qi) & 1ull) continue;
+            if (popc(S) < cap) {
+              const uint64_t S2 = S | (1ull << qi);
+              if (popc(S2 & ts.high) > ts.hcap) continue;
+              collect(true, S2, cand);
+              if (cand.gates.size() > best.gates.size()) {
+                std::swap(best, cand);
+                S = S2;
+                improved = true;
+              }
+              continue;
             }
-            continue;
-          }
-          for (int qo = lowb; qo < n; ++qo) {
-            if (!((S >> qo) & 1ull)) continue;
-            const uint64_t S2 = (S & ~(1ull << qo)) | (1ull << qi);
-            if (popc(S2 & ts.high) > ts.hcap) continue;
-            collect(true, S2, cand);
-            if (cand.gates.size() > pl.gates.size()) {
-              std::swap(pl, cand);
-              S = S2;
-              improved = true;
-              break;  // S changed: continue with the next qubit to bring in
+            for (int qo = lowb; qo < n; ++qo) {
+              if (!((S >> qo) & 1ull)) continue;
+              const uint64_t S2 = (S & ~(1ull << qo)) | (1ull << qi);
+              if (popc(S2 & ts.high) > ts.hcap) continue;
+              collect(true, S2, cand);
+              if (cand.gates.size() > best.gates.size()) {
+                std::swap(best, cand);
+                S = S2;
+                improved = true;
+                break;  // S changed: continue with the next qubit to bring in
+              }
             }
           }
+          if (!improved) break;
         }
-        if (!improved) break;
+      };
+      climb(pl.other, pl);
+      const int pool_q = n - lowb;
+      for (int k = 0; k < search_restarts && pool_q > cap; ++k) {
+        uint64_t rs = (uint64_t)*sweeps * 0x9E3779B97F4A7C15ull + (uint64_t)(k + 1) * 0xD1B54A32D192ED03ull;
+        uint64_t S = 0;
+        for (int tries = 0; popc(S) < cap && tries < 64 * cap; ++tries) {
+          rs += 0x9E3779B97F4A7C15ull;  // SplitMix64 step
+          uint64_t z = rs;
+          z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+          z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+          z ^= z >> 31;
+          const int q = lowb + (int)(z % (uint64_t)pool_q);
+          const uint64_t S2 = S | (1ull << q);
+          if (popc(S2 & ts.high) <= ts.hcap) S = S2;
+        }
+        collect(true, S, cur);
+        climb(S, cur);
+        if (cur.gates.size() > pl.gates.size()) std::swap(pl, cur);
       }
     }
     for (int gi : pl.gates) done[gi] = 1;
