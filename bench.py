@@ -247,7 +247,9 @@ def grover_bench(n: int, iterations: int, seed: int, only=None):
             entry["gatek_roofline"] = {
                 "kernel": "k_gatek<5> (dense 32 x 32 block on 5 qubits)",
                 "avg_ms": avg * 1e3, "achieved": byt / avg / 1e9, "unit": "GB/s",
-                "frac": byt / avg / HBM_PEAK, "tflops": flops / avg / 1e12}
+                "frac": byt / avg / HBM_PEAK, "tflops": flops / avg / 1e12,
+                "traffic": (recorded_traffic("k_gatek5_grover32.md")
+                            if dtype == "c128" and n == 32 else None)}
         res[name] = entry
         log(f"grover {name}: {len(opl) / dt:.1f} ops/s, err {err:.2e}")
         r.close()
@@ -360,7 +362,9 @@ def main() -> int:
                  "roofline": {"kernel": "k_sweep (fused sweep, complex64)",
                               "achieved": ach64 / 1e9, "unit": "GB/s",
                               "frac": ach64 / HBM_PEAK, "avg_ms": avg64 * 1e3,
-                              "launches": cnt64[0]}}
+                              "launches": cnt64[0],
+                              "traffic": (recorded_traffic("k_sweep_c64_30q.md")
+                                          if n == 30 else None)}}
         if reg is not None:
             entry["ratio_vs_c128"] = g64 / out["value"]
             entry["infidelity_vs_c128"] = r64.infidelity(reg)

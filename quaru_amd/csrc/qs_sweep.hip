@@ -204,40 +204,11 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
   ops.push_back(op);
 }
 
-// Compute units of the current device (cached per device).
-int device_cus() {
-  static int cus[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] == 0) {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        v <= 0)
-      v = 256;
-    cus[dev] = v;
-  }
-  return cus[dev];
-}
-
 template <typename R, int A>
 hipError_t launch_sweep(SweepRunner &run, const void *d_blob, uint64_t ntiles) {
   using V = typename SV2<R>::T;
   const uint64_t per_block = kThreads / kWave;
   const uint64_t grid = (ntiles + per_block - 1) / per_block;
-  if constexpr (std::is_same<R, double>::value && A == 4) {
-    static const int pipe = env_int("QS_SWEEP_PIPE", 0);
-    if (pipe > 0) {
-      // persistent waves: `pipe` workgroups per compute unit
-      const uint64_t pgrid = std::min<uint64_t>(grid, (uint64_t)device_cus() * (uint64_t)pipe);
-      if (run.n <= 32)
-        hipLaunchKernelGGL((k_sweep_pipe<A, uint32_t>), dim3((unsigned)pgrid), dim3(kThreads), 0,
-                           run.stream, (double2 *)run.psi, (const char *)d_blob);
-      else
-        hipLaunchKernelGGL((k_sweep_pipe<A, uint64_t>), dim3((unsigned)pgrid), dim3(kThreads), 0,
-                           run.stream, (double2 *)run.psi, (const char *)d_blob);
-      return hipGetLastError();
-    }
-  }
   if (run.n <= 32)
     hipLaunchKernelGGL((k_sweep<R, A, uint32_t>), dim3((unsigned)grid), dim3(kThreads), 0,
                        run.stream, (V *)run.psi, (const char *)d_blob);

@@ -42,10 +42,7 @@ constexpr int kThreads = 256;  // 4 waves = 4 independent tiles per workgroup
 constexpr int kWave = 64;
 constexpr int kLaneBits = 6;
 constexpr int kMaxA = 5;
-#ifndef QS_XCHG_BATCH
-#define QS_XCHG_BATCH 4
-#endif
-constexpr int kXchgBatch = QS_XCHG_BATCH;  // registers exchanged per scheduling group
+constexpr int kXchgBatch = 4;  // registers exchanged per scheduling group
 
 enum : uint32_t {
   OP_U1 = 1,      // uncontrolled 2x2 on one qubit (register slot or lane bit)
@@ -484,11 +481,7 @@ __device__ __forceinline__ I lane_offset(const QS_CONST SweepHeader &h, int lane
 
 // One wave per tile: load, apply every op, store.
 template <typename R, int A, typename I>
-#ifndef QS_SWEEP_WAVES
-#define QS_SWEEP_WAVES 1
-#endif
-__global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(QS_SWEEP_WAVES))) void k_sweep(typename SV2<R>::T *__restrict__ psi,
+__global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restrict__ psi,
                                                     const char *__restrict__ blob_g) {
   constexpr int T = kLaneBits + A;
   constexpr int NA = 1 << A;
@@ -521,104 +514,6 @@ __attribute__((amdgpu_waves_per_eu(QS_SWEEP_WAVES))) void k_sweep(typename SV2<R
     v.x = a.re[r];
     v.y = a.im[r];
     __builtin_nontemporal_store(v, dst + (c.gbase | goff<A>(g, r)));
-  }
-}
-
-// Software-pipelined sweep for complex128: persistent waves walk tiles
-// wave_id, wave_id + W, ... (W = waves in the grid) with two LDS buffers
-// per wave.  While the ops of tile t run in registers, tile t + W is already
-// on its way into the IN buffer (global_load_lds_dwordx4: lane l's 16 bytes
-// land at buffer + 16 l), and the results of tile t - W drain from the OUT
-// buffer to HBM.  Per tile:
-//   ops(t) in registers; registers -> OUT; IN -> registers (tile t + W);
-//   DMA tile t + 2W -> IN; OUT -> HBM (tile t).
-// The wait for the IN data comes one whole tile of arithmetic after its DMA
-// and after the previous tile's stores, so neither load nor store latency
-// sits on the wave's critical path.  Every lane reads back only the LDS slots
-// its own loads and writes filled, so no barrier is needed.
-typedef __attribute__((address_space(3))) void lds_void_t;
-
-template <int A, typename I>
-__device__ __forceinline__ void dma_tile(const double2 *psi, I base, const Regs<A, I> &g,
-                                         double2 *buf) {
-#pragma unroll
-  for (int r = 0; r < (1 << A); ++r)
-    __builtin_amdgcn_global_load_lds((const void *)(psi + (base | goff<A>(g, r))),
-                                     (lds_void_t *)(buf + r * kWave), 16, 0, 0);
-}
-
-template <int A, typename I>
-__global__ __launch_bounds__(kThreads) void k_sweep_pipe(double2 *__restrict__ psi,
-                                                         const char *__restrict__ blob_g) {
-  constexpr int T = kLaneBits + A;
-  constexpr int NA = 1 << A;
-  constexpr int W = kThreads / kWave;
-  __shared__ double2 lin[W][NA * kWave];
-  __shared__ double2 lout[W][NA * kWave];
-  const Blob<double> b = open_blob<double>(blob_g);
-  const QS_CONST SweepHeader &h = *b.h;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
-  double2 *in = lin[wv];
-  double2 *out = lout[wv];
-  const uint64_t stride = (uint64_t)gridDim.x * W;
-  uint64_t tile = (uint64_t)blockIdx.x * W + wv;
-  if (tile >= h.ntiles) return;  // whole waves only
-  const Regs<A, I> g = reg_bits<A, I>(h);
-  const I lo = lane_offset<I>(h, lane);
-  Ctx<I> c;
-  c.lane = lane;
-  c.gbase = (I)tile_base<T>(tile, h) | lo;
-  Amp<double, NA> a;
-  dma_tile<A>(psi, c.gbase, g, in);
-#pragma unroll
-  for (int r = 0; r < NA; ++r) {
-    const double2 v = in[r * kWave + lane];
-    a.re[r] = v.x;
-    a.im[r] = v.y;
-  }
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): IN read before its refill
-  uint64_t next = tile + stride;
-  I next_base = 0;
-  if (next < h.ntiles) {
-    next_base = (I)tile_base<T>(next, h) | lo;
-    dma_tile<A>(psi, next_base, g, in);
-  }
-  for (;;) {
-    run_ops<A>(a, b.ops, b.pool, (int)h.nops, c, g);
-#pragma unroll
-    for (int r = 0; r < NA; ++r) {
-      double2 v;
-      v.x = a.re[r];
-      v.y = a.im[r];
-      out[r * kWave + lane] = v;
-    }
-    const bool more = next < h.ntiles;
-    uint64_t after = next + stride;
-    if (more) {
-#pragma unroll
-      for (int r = 0; r < NA; ++r) {
-        const double2 v = in[r * kWave + lane];
-        a.re[r] = v.x;
-        a.im[r] = v.y;
-      }
-      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-      if (after < h.ntiles)
-        dma_tile<A>(psi, (I)tile_base<T>(after, h) | lo, g, in);
-    }
-    typename SNT<double>::T *dst = reinterpret_cast<typename SNT<double>::T *>(psi);
-#pragma unroll
-    for (int r = 0; r < NA; ++r) {
-      const double2 v = out[r * kWave + lane];
-      typename SNT<double>::T w;
-      w.x = v.x;
-      w.y = v.y;
-      __builtin_nontemporal_store(w, dst + (c.gbase | goff<A>(g, r)));
-    }
-    if (!more) break;
-    c.gbase = next_base;
-    next = after;
-    if (next < h.ntiles) next_base = (I)tile_base<T>(next, h) | lo;
   }
 }
 
