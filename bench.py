@@ -345,6 +345,8 @@ def main() -> int:
             "avg_ms": sweep_avg_s * 1e3, "bytes_per_launch": sweep_bytes}
         out["sweeps_per_step"] = st["sweeps"] / args.steps
         out["gates_per_sweep"] = G * args.steps / max(st["sweeps"], 1)
+        out["ops_per_sweep"] = st["sweep_ops"] / max(st["sweeps"], 1)
+        out["merged_per_step"] = st["merged"] / args.steps
         out["kernel_time_fraction"] = sum(ms) / 1e3 / dt
         out["effective_gb_per_s"] = alg_bytes / dt / 1e9
         out["effective_note"] = ("sum over gates of 2*|changed set|*16 B per wall second; fusion "
@@ -354,7 +356,7 @@ def main() -> int:
     # ---- complex64 ----
     if sections["c64"]:
         r64 = Register.new([False] * n, "c64")
-        dt64, ms64, cnt64, _ = timed_run(r64, w.ops, args.steps, args.warmup)
+        dt64, ms64, cnt64, st64 = timed_run(r64, w.ops, args.steps, args.warmup)
         avg64 = ms64[0] / 1e3 / max(cnt64[0], 1)
         ach64 = 2.0 * (1 << n) * 8 / avg64 if cnt64[0] else 0.0
         g64 = G * args.steps / dt64
@@ -364,7 +366,9 @@ def main() -> int:
                               "frac": ach64 / HBM_PEAK, "avg_ms": avg64 * 1e3,
                               "launches": cnt64[0],
                               "traffic": (recorded_traffic("k_sweep_c64_30q.md")
-                                          if n == 30 else None)}}
+                                          if n == 30 else None)},
+                 "sweeps_per_step": st64["sweeps"] / args.steps,
+                 "ops_per_sweep": st64["sweep_ops"] / max(st64["sweeps"], 1)}
         if reg is not None:
             entry["ratio_vs_c128"] = g64 / out["value"]
             entry["infidelity_vs_c128"] = r64.infidelity(reg)

@@ -178,6 +178,13 @@ int qs_set_fusion(qs_register *r, int enabled);
 int qs_stats(qs_register *r, uint64_t *sweeps, uint64_t *gate_kernels, uint64_t *gates,
              int reset);
 
+/* Sweep op statistics since the last reset: op records executed by the
+ * fused sweep kernel, and gates merged into a neighbouring one-qubit op of
+ * their sweep (a CNOT / Toffoli target, controlled 2x2 or CZ-type phase next
+ * to a 2x2 on the same qubit becomes one multiplexed 2x2).  No reference
+ * counterpart: engine diagnostics. */
+int qs_stats_sweep_ops(qs_register *r, uint64_t *ops, uint64_t *merged, int reset);
+
 /* Kernel classes for qs_timing_read. */
 enum qs_kernel_class {
     QS_K_SWEEP = 0,  /* fused sweep (a run of gates in one pass over the state) */

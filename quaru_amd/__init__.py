@@ -11,10 +11,11 @@ from .operation import (Operation, QuantumCircuit, cnot, cnx, cnz, cz, hadamard,
                         hadamard_transform, identity, not_, pauli_y, pauli_z, phase, rx, ry, rz,
                         swap, to_controlled, to_quantum_gate, u)
 from .register import EngineError, OperationError, QuaruPanic, Register
+from . import openqasm
 
 __all__ = [
     "Register", "Operation", "QuantumCircuit", "OperationError", "QuaruPanic", "EngineError",
-    "operation", "identity", "hadamard", "hadamard_transform", "cnot", "to_quantum_gate",
+    "operation", "openqasm", "identity", "hadamard", "hadamard_transform", "cnot", "to_quantum_gate",
     "to_controlled", "swap", "phase", "not_", "pauli_y", "pauli_z", "cnx", "cnz", "cz", "u",
     "rx", "ry", "rz",
 ]

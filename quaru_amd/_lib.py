@@ -63,6 +63,7 @@ SIGNATURES = {
     "qs_sync": (_I, [_P]),
     "qs_set_fusion": (_I, [_P, _I]),
     "qs_stats": (_I, [_P, _U64P, _U64P, _U64P, _I]),
+    "qs_stats_sweep_ops": (_I, [_P, _U64P, _U64P, _I]),
     "qs_timing_enable": (_I, [_P, _I]),
     "qs_timing_read": (_I, [_P, _P, _P, _I]),
     "qs_pcircuit_new": (_I, [_S, _PP]),

@@ -83,6 +83,7 @@ struct qs_register {
   bool fusion = true;
   std::vector<Gate> pending;
   uint64_t st_sweeps = 0, st_kernels = 0, st_gates = 0;
+  uint64_t st_sweep_ops = 0, st_merged = 0;
   qs::SweepEngine *engine = nullptr;
   // kernel timing: event pairs recorded on `stream` around each launch
   bool timing = false;
@@ -470,6 +471,8 @@ int flush(qs_register *r) {
     };
     run.single = [r](const Gate &g) { return launch_gate(r, g); };
     if (r->timing) run.mark = [r](bool begin) { mark(r, QS_K_SWEEP, begin); };
+    run.sweep_ops = &r->st_sweep_ops;
+    run.merged = &r->st_merged;
     e = qs::sweep_execute(r->engine, run, gates, &r->st_sweeps, &r->st_kernels);
   } else {
     for (const Gate &g : gates) {
@@ -1209,6 +1212,16 @@ int qs_stats(qs_register *r, uint64_t *sweeps, uint64_t *gate_kernels, uint64_t 
   if (gate_kernels) *gate_kernels = r->st_kernels;
   if (gates) *gates = r->st_gates;
   if (reset) r->st_sweeps = r->st_kernels = r->st_gates = 0;
+  return QS_OK;
+}
+
+int qs_stats_sweep_ops(qs_register *r, uint64_t *ops, uint64_t *merged, int reset) {
+  if (!r) return fail(QS_ERR_ARGUMENT, 0, 0, "register is NULL");
+  int rc = flush(r);
+  if (rc) return rc;
+  if (ops) *ops = r->st_sweep_ops;
+  if (merged) *merged = r->st_merged;
+  if (reset) r->st_sweep_ops = r->st_merged = 0;
   return QS_OK;
 }
 

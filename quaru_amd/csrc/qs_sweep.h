@@ -42,6 +42,14 @@ struct SweepRunner {
   // Called with true before and false after each sweep kernel launch
   // (timing hooks); may be empty.
   std::function<void(bool)> mark;
+  // Op records run by sweep kernels, and gates merged into another op of
+  // their sweep; incremented when non-null.
+  uint64_t *sweep_ops = nullptr;
+  uint64_t *merged = nullptr;
+  // When set, each sweep's descriptor (the host bytes the kernel would read)
+  // goes to this function instead of being uploaded and launched: the CPU
+  // checks of the planner run the sweeps on a host copy of the state.
+  std::function<hipError_t(const void *, size_t)> host_sweep;
 };
 
 // Apply `gates` in order (with commuting reorders) through sweeps and
@@ -61,6 +69,14 @@ int sweep_low_bits(bool c64);
 // Tile qubits at or above sweep_page_qubit() each double the pages a wave
 // touches; at most A + sweep_high_lanes() of them go into one sweep.
 int sweep_page_qubit(bool c64);
+
+// Whether one-qubit ops of a sweep are merged into multiplexed 2x2 ops
+// (QS_SWEEP_MERGE, default on).
+bool sweep_merge_ops();
+
+// Whether sweep launches give each XCD one contiguous range of tiles
+// (QS_SWEEP_XCD, default on).
+bool sweep_xcd_chunks();
 int sweep_high_lanes();
 
 }  // namespace qs

@@ -26,11 +26,14 @@
 
 #include "qs_sweep.h"
 #include "qs_sweep_kernel.h"
+#include "qs_sweep_merge.h"
 
 namespace qs {
 namespace {
 
 using namespace qs::sweep;
+using namespace qs::merge;
+static_assert(kFirstLaneSlot == (int)kLaneSlot, "slot numbering shared with the merge rules");
 
 constexpr int kMaxOps = 512;     // op records per sweep
 constexpr int kWindow = 384;     // look-ahead over unscheduled gates
@@ -204,17 +207,43 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
   ops.push_back(op);
 }
 
+template <typename R>
+void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, std::vector<OpRec> &ops,
+                std::vector<R> &pool) {
+  OpRec op{};
+  op.s0 = (uint32_t)slot[o.t];
+  op.moff = (uint32_t)pool.size();
+  auto put = [&](const cd &v) {
+    pool.push_back((R)v.re);
+    pool.push_back((R)v.im);
+  };
+  if (o.pm == 0) {
+    op.type = OP_U1;
+    for (int k = 0; k < 4; ++k) put(o.m1[k]);
+    op.code = op_code(op.type, op.s0, 0);
+  } else {
+    op.type = OP_MUX;
+    op.cmask = o.pm;
+    op.cval = o.pv;
+    for (int k = 0; k < 4; ++k) put(o.m0[k]);
+    for (int k = 0; k < 4; ++k) put(o.m1[k]);
+    op.code = op_code(op.type, op.s0, 0, (o.pm & reg_qubits) != 0);
+  }
+  ops.push_back(op);
+}
+
 template <typename R, int A>
 hipError_t launch_sweep(SweepRunner &run, const void *d_blob, uint64_t ntiles) {
   using V = typename SV2<R>::T;
   const uint64_t per_block = kThreads / kWave;
   const uint64_t grid = (ntiles + per_block - 1) / per_block;
+  const uint32_t xcd_chunk = (sweep_xcd_chunks() && grid % 8 == 0) ? (uint32_t)(grid / 8) : 0u;
   if (run.n <= 32)
     hipLaunchKernelGGL((k_sweep<R, A, uint32_t>), dim3((unsigned)grid), dim3(kThreads), 0,
-                       run.stream, (V *)run.psi, (const char *)d_blob);
+                       run.stream, (V *)run.psi, (const char *)d_blob, xcd_chunk);
   else
     hipLaunchKernelGGL((k_sweep<R, A, uint64_t>), dim3((unsigned)grid), dim3(kThreads), 0,
-                       run.stream, (V *)run.psi, (const char *)d_blob);
+                       run.stream, (V *)run.psi, (const char *)d_blob, xcd_chunk);
   return hipGetLastError();
 }
 
@@ -427,8 +456,23 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     }
     ops.clear();
     pool.clear();
-    for (int gi : pl.gates) encode_gate<R>(gates[gi], slot, ops, pool);
+    std::vector<SweepOp> items(pl.gates.size());
+    for (size_t k = 0; k < pl.gates.size(); ++k) items[k].g = &gates[pl.gates[k]];
+    uint64_t reg_qubits = 0;
+    for (int j = 0; j < A; ++j) reg_qubits |= 1ull << regs[j];
+    if (sweep_merge_ops()) merge_sweep_ops(items, slot, reg_qubits);
+    for (const SweepOp &o : items) {
+      if (!o.alive) {
+        if (run.merged) ++*run.merged;
+        continue;
+      }
+      if (o.g)
+        encode_gate<R>(*o.g, slot, ops, pool);
+      else
+        encode_mux<R>(o, slot, reg_qubits, ops, pool);
+    }
     h.nops = (uint32_t)ops.size();
+    if (run.sweep_ops) *run.sweep_ops += ops.size();
     h.npool = (uint32_t)pool.size();
     h.ntiles = 1ull << (n - T);
     const size_t hbytes = (sizeof(SweepHeader) + 15) & ~(size_t)15;
@@ -439,6 +483,12 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     if (!pool.empty())
       std::memcpy(blob.data() + hbytes + ops.size() * sizeof(OpRec), pool.data(),
                   pool.size() * sizeof(R));
+    if (run.host_sweep) {
+      hipError_t e = run.host_sweep(blob.data(), blob.size());
+      if (e != hipSuccess) return e;
+      ++*sweeps;
+      continue;
+    }
     void *d_blob = nullptr;
     hipError_t e = run.upload(blob.data(), blob.size(), &d_blob);
     if (e != hipSuccess) return e;
@@ -477,6 +527,18 @@ int sweep_high_lanes() { return std::max(0, env_int("QS_SWEEP_HIGH_LANES", kLane
 int sweep_low_bits(bool c64) {
   const int b = env_int(c64 ? "QS_SWEEP_LOW_C64" : "QS_SWEEP_LOW_C128", 4);
   return std::min(kLaneBits, std::max(0, b));
+}
+
+// Merging of one-qubit ops inside a sweep (QS_SWEEP_MERGE, default on).
+bool sweep_merge_ops() {
+  static const bool on = env_int("QS_SWEEP_MERGE", 1) != 0;
+  return on;
+}
+
+// Contiguous tile ranges per XCD (see xcd_block in qs_sweep_kernel.h).
+bool sweep_xcd_chunks() {
+  static const bool on = env_int("QS_SWEEP_XCD", 1) != 0;
+  return on;
 }
 
 int sweep_reg_bits(bool c64) {

@@ -50,7 +50,8 @@ enum : uint32_t {
   OP_X1 = 3,      // (controlled) exchange: Pauli X / CNOT / Toffoli targets
   OP_DENSE2 = 4,  // 4x4 on two register slots (matrix bit 0 <-> s0)
   OP_SPARSE = 5,  // multiply by pool value where (idx & cmask) == cval
-  OP_DIAG = 6     // multiply by pool[pattern of 1-2 act bits] where (idx & cmask) == cval
+  OP_DIAG = 6,    // multiply by pool[pattern of 1-2 act bits] where (idx & cmask) == cval
+  OP_MUX = 7      // 2x2 on one qubit: pool[0..3] where (idx & cmask) != cval, pool[4..7] where ==
 };
 
 // Slot encoding in s0/s1: 0..A-1 register slot, kLaneSlot + k lane bit k.
@@ -66,12 +67,21 @@ enum : uint32_t {
   CODE_X1 = CODE_C1 + kMaxA + 1,  // 13 .. 18
   CODE_SPARSE = CODE_X1 + kMaxA + 1,
   CODE_DIAG = CODE_SPARSE + 1,
-  CODE_DENSE2 = CODE_DIAG + 1     // + s0 * kMaxA + s1
+  CODE_DENSE2 = CODE_DIAG + 1,    // + s0 * kMaxA + s1
+  // MUX whose pattern has no register-slot bit: the matrix is chosen once
+  // per lane.  + slot (kMaxA = any lane bit)
+  CODE_MUXL = CODE_DENSE2 + kMaxA * kMaxA,
+  // MUX on a register slot whose pattern lies on register slots only: the
+  // matrix is chosen per register pair, uniformly over the wave.  + slot
+  CODE_MUXR = CODE_MUXL + kMaxA + 1
 };
 
-__host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t s1) {
+// `reg_pattern`: the MUX pattern lies on register slots only (else on none).
+__host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t s1,
+                                            bool reg_pattern = false) {
   const uint32_t j0 = s0 >= kLaneSlot ? (uint32_t)kMaxA : s0;
   switch (type) {
+    case OP_MUX: return (reg_pattern ? CODE_MUXR : CODE_MUXL) + j0;
     case OP_U1: return CODE_U1 + j0;
     case OP_C1: return CODE_C1 + j0;
     case OP_X1: return CODE_X1 + j0;
@@ -275,6 +285,95 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
   }
 }
 
+// (u, w) = t * (x, y) for the 2x2 `t` = {t0 .. t3} in row-major order.
+template <typename R>
+__device__ __forceinline__ void mat2(R &ur, R &ui, R &wr, R &wi, const R *tr, const R *ti, R xr,
+                                     R xi, R yr, R yi) {
+  ur = 0;
+  ui = 0;
+  wr = 0;
+  wi = 0;
+  cfma(ur, ui, tr[0], ti[0], xr, xi);
+  cfma(ur, ui, tr[1], ti[1], yr, yi);
+  cfma(wr, wi, tr[2], ti[2], xr, xi);
+  cfma(wr, wi, tr[3], ti[3], yr, yi);
+}
+
+// Multiplexed 2x2 on register slot J: matrix m[0..3] where the op's pattern
+// fails, m[4..7] where it holds.  kRegPattern: the pattern lies on register
+// slots only, so the choice is uniform per register pair (scalar selects);
+// otherwise it lies on no register slot and each lane chooses once.
+template <int J, bool kRegPattern, int A, typename R, typename I>
+__device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                           const QS_CONST R *m, const Ctx<I> &c,
+                                           const Regs<A, I> &g) {
+  const Pred<I> p = make_pred<A>(op, c, g);
+  constexpr int bj = 1 << J;
+  R lr[4], li[4];
+  if constexpr (!kRegPattern) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      lr[k] = p.lane_ok ? m[8 + 2 * k] : m[2 * k];
+      li[k] = p.lane_ok ? m[9 + 2 * k] : m[2 * k + 1];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < (1 << A); ++r) {
+    if ((r >> J) & 1) continue;
+    R tr[4], ti[4];
+    if constexpr (kRegPattern) {
+      const QS_CONST R *mm = ((goff<A>(g, r) & p.cmr) == p.cvr) ? m + 8 : m;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        tr[k] = mm[2 * k];
+        ti[k] = mm[2 * k + 1];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        tr[k] = lr[k];
+        ti[k] = li[k];
+      }
+    }
+    R ur, ui, wr, wi;
+    mat2(ur, ui, wr, wi, tr, ti, a.re[r], a.im[r], a.re[r | bj], a.im[r | bj]);
+    a.re[r] = ur;
+    a.im[r] = ui;
+    a.re[r | bj] = wr;
+    a.im[r | bj] = wi;
+  }
+}
+
+// Multiplexed 2x2 on a lane qubit; the pattern has no register-slot bit, so
+// each lane picks its matrix row once for the whole op.
+template <int A, typename R, typename I>
+__device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                            const QS_CONST R *m, const Ctx<I> &c,
+                                            const Regs<A, I> &g) {
+  const Pred<I> p = make_pred<A>(op, c, g);
+  const int k = (int)op.s0 - (int)kLaneSlot;
+  const bool hi = (c.lane >> k) & 1;
+  const int addr = (c.lane ^ (1 << k)) << 2;
+  // this lane's output row: the coefficient of its own amplitude (s) and of
+  // the partner's (p); selects keep the matrix on scalar loads
+  const bool on = p.lane_ok;
+  const R sr = hi ? (on ? m[14] : m[6]) : (on ? m[8] : m[0]);
+  const R si = hi ? (on ? m[15] : m[7]) : (on ? m[9] : m[1]);
+  const R pr = hi ? (on ? m[12] : m[4]) : (on ? m[10] : m[2]);
+  const R pi = hi ? (on ? m[13] : m[5]) : (on ? m[11] : m[3]);
+#pragma unroll
+  for (int r = 0; r < (1 << A); ++r) {
+    const R xr = a.re[r], xi = a.im[r];
+    const R yr = xchg(xr, addr), yi = xchg(xi, addr);
+    R ur = 0, ui = 0;
+    cfma(ur, ui, sr, si, xr, xi);
+    cfma(ur, ui, pr, pi, yr, yi);
+    a.re[r] = ur;
+    a.im[r] = ui;
+    if ((r % kXchgBatch) == kXchgBatch - 1) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <int J, int A, typename R, typename I>
 __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                           const Ctx<I> &c, const Regs<A, I> &g) {
@@ -431,6 +530,17 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       QS_REG2_ROW(2)
       QS_REG2_ROW(3)
       QS_REG2_ROW(4)
+      QS_REG1(CODE_MUXL, 0, (op_mux_reg<0, false, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXL, 1, (op_mux_reg<1, false, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXL, 2, (op_mux_reg<2, false, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXL, 3, (op_mux_reg<3, false, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXL, 4, (op_mux_reg<4, false, A>(a, op, m, c, g)))
+      case CODE_MUXL + kMaxA: op_mux_lane<A>(a, op, m, c, g); break;
+      QS_REG1(CODE_MUXR, 0, (op_mux_reg<0, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXR, 1, (op_mux_reg<1, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXR, 2, (op_mux_reg<2, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXR, 3, (op_mux_reg<3, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXR, 4, (op_mux_reg<4, true, A>(a, op, m, c, g)))
       default: break;
     }
   }
@@ -479,16 +589,28 @@ __device__ __forceinline__ I lane_offset(const QS_CONST SweepHeader &h, int lane
   return lo;
 }
 
+// Workgroups go to the 8 XCDs round robin (workgroup w to XCD w mod 8).
+// With `xcd_chunk` = grid / 8 the workgroup order is permuted so that each
+// XCD walks one contiguous eighth of the tiles: its L2 and address
+// translation caches then see the pages of one region instead of every
+// region at once.  xcd_chunk = 0 keeps the plain order.
+__device__ __forceinline__ uint64_t xcd_block(uint32_t xcd_chunk) {
+  const uint32_t w = blockIdx.x;
+  if (xcd_chunk == 0) return w;
+  return (uint64_t)(w & 7u) * xcd_chunk + (w >> 3);
+}
+
 // One wave per tile: load, apply every op, store.
 template <typename R, int A, typename I>
 __global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restrict__ psi,
-                                                    const char *__restrict__ blob_g) {
+                                                    const char *__restrict__ blob_g,
+                                                    uint32_t xcd_chunk) {
   constexpr int T = kLaneBits + A;
   constexpr int NA = 1 << A;
   const Blob<R> b = open_blob<R>(blob_g);
   const QS_CONST SweepHeader &h = *b.h;
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / kWave) + (threadIdx.x / kWave);
+  const uint64_t tile = xcd_block(xcd_chunk) * (kThreads / kWave) + (threadIdx.x / kWave);
   if (tile >= h.ntiles) return;  // whole waves only
   const Regs<A, I> g = reg_bits<A, I>(h);
   Ctx<I> c;

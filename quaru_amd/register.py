@@ -398,7 +398,11 @@ class Register:
         s, k, g = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
         _check(_lib.load().qs_stats(self._h, ctypes.byref(s), ctypes.byref(k), ctypes.byref(g),
                                     1 if reset else 0))
-        return {"sweeps": s.value, "gate_kernels": k.value, "gates": g.value}
+        o, m = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(_lib.load().qs_stats_sweep_ops(self._h, ctypes.byref(o), ctypes.byref(m),
+                                              1 if reset else 0))
+        return {"sweeps": s.value, "gate_kernels": k.value, "gates": g.value,
+                "sweep_ops": o.value, "merged": m.value}
 
     # -- printing (src/register.rs:379-401) --------------------------------------
     def print_probabilities(self) -> None:

@@ -196,6 +196,54 @@ def test_random_circuit_sweeps(n, layers, triples):
             assert st["gates"] == len(w.ops)
 
 
+def merge_stress_ops(rng, n, count):
+    """Runs of one-qubit gates mixed with the gates a sweep merges into them
+    (CNOT / Toffoli targets, controlled 2x2s, controlled phases, CZ / CCZ),
+    with targets and controls on every kind of tile position."""
+    opl = []
+    for _ in range(count):
+        q = [int(x) for x in rng.permutation(n)[:4]]
+        kind = rng.integers(0, 10)
+        if kind < 3:
+            opl.append(ops.Operation(random_unitary(rng, 2), [q[0]]))
+        elif kind == 3:
+            opl.append(ops.cnot(q[1], q[0]))
+        elif kind == 4:
+            opl.append(ops.cnx([q[1], q[2]], q[0]))
+        elif kind == 5:
+            opl.append(ops.cnz([q[1], q[2]], q[0]))
+        elif kind == 6:
+            opl.append(ops.cz(q[0], q[1]))
+        elif kind == 7:
+            opl.append(ops.to_controlled(ops.Operation(random_unitary(rng, 2), [q[0]]), q[1]))
+        elif kind == 8:
+            opl.append(ops.to_controlled(ops.phase(q[0]), q[1]))
+        else:
+            opl.append(ops.rz(float(rng.uniform(0, 6)), q[0]))
+    return opl
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [11, 14])
+def test_sweep_merged_ops_match_oracle(n, dtype):
+    """Merged multiplexed ops inside sweeps give the oracle's result."""
+    rng = np.random.default_rng(77 + n)
+    opl = merge_stress_ops(rng, n, 400)
+    psi = random_state(rng, n)
+    ref = oracle_run(psi.copy(), opl)
+    r = Register.new([False] * n, dtype)
+    r.state = psi
+    r.stats(reset=True)
+    for op in opl:
+        r.apply(op)
+    st = r.stats()
+    assert st["merged"] > 40 and st["sweeps"] > 0
+    if dtype == "c128":
+        assert maxdiff(r.state, ref) < TOL
+    else:
+        assert infid(r.state, ref) < 1e-6
+
+
 def test_golden_config1_on_gpu(golden):
     g = golden("c1_12q.npz")
     w = circuits.random_circuit(12, 20, seed=int(g["seed"]))

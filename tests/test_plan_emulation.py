@@ -1,0 +1,104 @@
+"""The sweep planner end to end on the CPU: the engine's classifier and
+planner produce the exact sweep descriptors the GPU would run, and
+tests/cpp/plan_emulator.cpp executes them on a host state the way k_sweep
+decodes them (op code dispatch, slot tables, per-form pattern bits).  The
+result must match the oracle to 1e-12 (descriptors of complex128 registers)
+or to float rounding (complex64 descriptors, whose matrices are float).
+
+This pins the planner's gate ordering, the op merging, the slot encoding and
+the descriptor layout without a GPU; the kernels' arithmetic is pinned by the
+GPU tests.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from quaru_amd import circuits
+from quaru_amd import operation as ops
+from tests.conftest import random_state, random_unitary
+from tests.test_gpu_register import gate_zoo, merge_stress_ops
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "quaru_amd", "csrc")
+LIBDIR = os.path.join(ROOT, "quaru_amd")
+SRC = os.path.join(ROOT, "tests", "cpp", "plan_emulator.cpp")
+HIPCC = "/opt/rocm/bin/hipcc"
+
+pytestmark = pytest.mark.skipif(
+    not (os.path.exists(HIPCC) and os.path.exists(os.path.join(LIBDIR, "libquaru_hip.so"))),
+    reason="needs hipcc and the built library")
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("emu") / "libplan_emulator.so")
+    subprocess.run([HIPCC, "-x", "hip", "--offload-arch=gfx950", "-std=c++17", "-O1", "-fPIC",
+                    "-shared", "-I", CSRC, SRC, "-L", LIBDIR, "-lquaru_hip",
+                    f"-Wl,-rpath,{LIBDIR}", "-o", out],
+                   check=True, capture_output=True, timeout=600)
+    lib = ctypes.CDLL(out)
+    lib.emu_run.restype = ctypes.c_int
+    lib.emu_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                            ctypes.c_char_p, ctypes.c_int]
+
+    def run(n, dtype, psi, opl):
+        st = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+        mats = np.ascontiguousarray(np.concatenate(
+            [np.asarray(op.matrix(), dtype=np.complex128).reshape(-1) for op in opl]))
+        tg = np.ascontiguousarray(np.concatenate(
+            [np.asarray(op.targets(), dtype=np.uint64) for op in opl]), dtype=np.uint64)
+        ks = np.ascontiguousarray([len(op.targets()) for op in opl], dtype=np.int32)
+        stats = np.zeros(4, dtype=np.uint64)
+        err = ctypes.create_string_buffer(512)
+        rc = lib.emu_run(n, 1 if dtype == "c64" else 0, st.ctypes.data, len(opl),
+                         mats.ctypes.data, tg.ctypes.data, ks.ctypes.data, stats.ctypes.data,
+                         err, 512)
+        assert rc == 0, err.value.decode()
+        return st, dict(zip(["sweeps", "singles", "sweep_ops", "merged"], map(int, stats)))
+    return run
+
+
+def oracle_run(psi, opl):
+    for op in opl:
+        psi = oracle.apply_fast(psi, op.matrix(), op.targets())
+    return psi
+
+
+TOL = {"c128": 1e-12, "c64": 2e-6}
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [11, 14, 16])
+def test_merge_stress_plans(emu, n, dtype):
+    rng = np.random.default_rng(77 + n)
+    opl = merge_stress_ops(rng, n, 400)
+    psi = random_state(rng, n)
+    got, st = emu(n, dtype, psi, opl)
+    ref = oracle_run(psi.copy(), opl)
+    assert st["sweeps"] > 0 and st["merged"] > 40
+    assert np.max(np.abs(got - ref)) < TOL[dtype]
+
+
+@pytest.mark.parametrize("n,layers,triples", [(12, 4, 0), (14, 3, 4), (16, 2, 8), (17, 2, 8)])
+def test_random_circuit_plans(emu, n, layers, triples):
+    w = circuits.random_circuit(n, layers, seed=n * 31 + layers, triples=triples, measure=False)
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1
+    got, st = emu(n, "c128", psi, w.ops)
+    ref = oracle_run(psi.copy(), w.ops)
+    assert st["sweeps"] > 0
+    assert np.max(np.abs(got - ref)) < TOL["c128"]
+
+
+@pytest.mark.parametrize("n", [10, 13])
+def test_gate_zoo_plans(emu, n):
+    rng = np.random.default_rng(300 + n)
+    psi = random_state(rng, n)
+    zoo = gate_zoo(rng, n) + gate_zoo(rng, n)
+    got, _ = emu(n, "c128", psi, zoo)
+    assert np.max(np.abs(got - oracle_run(psi.copy(), zoo))) < TOL["c128"]
