@@ -218,6 +218,22 @@ __device__ __forceinline__ void cfma(R &ore, R &oim, R mr, R mi, R xr, R xi) {
   oim = fma(mi, xr, oim);
 }
 
+// A descriptor value read on a scalar load.  readfirstlane keeps the value
+// opaque: a per-lane choice between two descriptor values then stays a
+// select of two scalars.  Without it the compiler turns `c ? m[6] : m[0]`
+// into one vector load from a per-lane address, a memory round trip inside
+// the op loop.
+__device__ __forceinline__ double sld(const QS_CONST double *p) {
+  const long long v = *(const QS_CONST long long *)p;
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float sld(const QS_CONST float *p) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(*(const QS_CONST int *)p));
+}
+
 // 2x2 on register slot J; kCtl selects the controlled form.
 template <int J, bool kCtl, int A, typename R, typename I>
 __device__ __forceinline__ void op_1q_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
@@ -260,8 +276,8 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
   const int k = (int)op.s0 - (int)kLaneSlot;
   const bool hi = (c.lane >> k) & 1;
   const int addr = (c.lane ^ (1 << k)) << 2;
-  const R sr = hi ? m[6] : m[0], si = hi ? m[7] : m[1];
-  const R pr = hi ? m[4] : m[2], pi = hi ? m[5] : m[3];
+  const R sr = hi ? sld(m + 6) : sld(m + 0), si = hi ? sld(m + 7) : sld(m + 1);
+  const R pr = hi ? sld(m + 4) : sld(m + 2), pi = hi ? sld(m + 5) : sld(m + 3);
   Pred<I> p{};
   if constexpr (kCtl) p = make_pred<A>(op, c, g);
 #pragma unroll
@@ -313,8 +329,8 @@ __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpR
   if constexpr (!kRegPattern) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      lr[k] = p.lane_ok ? m[8 + 2 * k] : m[2 * k];
-      li[k] = p.lane_ok ? m[9 + 2 * k] : m[2 * k + 1];
+      lr[k] = p.lane_ok ? sld(m + 8 + 2 * k) : sld(m + 2 * k);
+      li[k] = p.lane_ok ? sld(m + 9 + 2 * k) : sld(m + 2 * k + 1);
     }
   }
 #pragma unroll
@@ -357,10 +373,10 @@ __device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST Op
   // this lane's output row: the coefficient of its own amplitude (s) and of
   // the partner's (p); selects keep the matrix on scalar loads
   const bool on = p.lane_ok;
-  const R sr = hi ? (on ? m[14] : m[6]) : (on ? m[8] : m[0]);
-  const R si = hi ? (on ? m[15] : m[7]) : (on ? m[9] : m[1]);
-  const R pr = hi ? (on ? m[12] : m[4]) : (on ? m[10] : m[2]);
-  const R pi = hi ? (on ? m[13] : m[5]) : (on ? m[11] : m[3]);
+  const R sr = hi ? (on ? sld(m + 14) : sld(m + 6)) : (on ? sld(m + 8) : sld(m + 0));
+  const R si = hi ? (on ? sld(m + 15) : sld(m + 7)) : (on ? sld(m + 9) : sld(m + 1));
+  const R pr = hi ? (on ? sld(m + 12) : sld(m + 4)) : (on ? sld(m + 10) : sld(m + 2));
+  const R pi = hi ? (on ? sld(m + 13) : sld(m + 5)) : (on ? sld(m + 11) : sld(m + 3));
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const R xr = a.re[r], xi = a.im[r];
@@ -462,13 +478,16 @@ __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec 
   const Pred<I> p = make_pred<A>(op, c, g);
   const int na = op.na;
   const int q0 = op.act[0], q1 = na > 1 ? op.act[1] : op.act[0];
+  R dv[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dv[k] = (k < 4 || na > 1) ? sld(m + k) : (R)0;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const I gi = c.gbase | goff<A>(g, r);
     const bool b0 = (gi >> q0) & 1u;
     const bool b1 = na > 1 && ((gi >> q1) & 1u);
-    const R vr = b1 ? (b0 ? m[6] : m[4]) : (b0 ? m[2] : m[0]);
-    const R vi = b1 ? (b0 ? m[7] : m[5]) : (b0 ? m[3] : m[1]);
+    const R vr = b1 ? (b0 ? dv[6] : dv[4]) : (b0 ? dv[2] : dv[0]);
+    const R vi = b1 ? (b0 ? dv[7] : dv[5]) : (b0 ? dv[3] : dv[1]);
     const bool on = pred_on<A>(p, g, r);
     const R xr = a.re[r], xi = a.im[r];
     R sr = 0, si = 0;
