@@ -394,6 +394,46 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
         if (cur.gates.size() > pl.gates.size()) std::swap(pl, cur);
       }
     }
+    // ---- hold back a plain 2x2 on qubit t whose next gate on t (a CNOT or
+    // Toffoli target, a controlled 2x2, a phase pattern, another 2x2) is
+    // left for a later sweep: nothing else of this sweep touches t after it,
+    // and the later sweep, which has t on its tile anyway, merges the two
+    // into one op.
+    if (sweep_merge_ops() && pl.gates.size() > 1) {
+      std::vector<char> in_sweep_last(64, 0);
+      std::vector<int> keep;
+      keep.reserve(pl.gates.size());
+      uint64_t later_touch = 0;  // qubits touched by gates after the current one in this sweep
+      std::vector<char> drop(pl.gates.size(), 0);
+      for (int k = (int)pl.gates.size() - 1; k >= 0; --k) {
+        const int gi = pl.gates[k];
+        const Gate &g = gates[gi];
+        cd u[4];
+        const uint64_t touch = gx[gi] | gz[gi];
+        if (plain_2x2(g, u) && !(later_touch & (1ull << g.act[0]))) {
+          const uint64_t bt = 1ull << g.act[0];
+          // the next gate on t outside this sweep
+          for (size_t i = (size_t)gi + 1; i < G; ++i) {
+            if (done[i] || !((gx[i] | gz[i]) & bt)) continue;
+            if (std::find(pl.gates.begin(), pl.gates.end(), (int)i) != pl.gates.end()) break;
+            int tt;
+            uint64_t pm, pv;
+            cd mt[4];
+            bool mergeable = plain_2x2(gates[i], mt);
+            for (int choice = 0; choice < 8 && !mergeable; ++choice) {
+              if (!pattern_2x2(gates[i], choice, tt, pm, pv, mt)) break;
+              mergeable = tt == g.act[0];
+            }
+            if (mergeable) drop[k] = 1;
+            break;
+          }
+        }
+        later_touch |= touch;
+      }
+      for (size_t k = 0; k < pl.gates.size(); ++k)
+        if (!drop[k]) keep.push_back(pl.gates[k]);
+      if (!keep.empty() && keep.size() < pl.gates.size()) pl.gates.swap(keep);
+    }
     for (int gi : pl.gates) done[gi] = 1;
     // ---- sweep or single kernels?
     double single_cost = 0.0;

@@ -73,6 +73,8 @@ bool fail(const std::string &m) {
   return false;
 }
 
+uint64_t *g_hist = nullptr;  // op count per code, when set
+
 template <typename R>
 bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) {
   SweepHeader h;
@@ -101,12 +103,14 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
   }
   if (smask != tmask || __builtin_popcountll(tmask) != T) return fail("tile tables disagree");
   if (h.ntiles != (1ull << (n - T))) return fail("tile count");
-  const uint64_t N = 1ull << n;
+  const bool plan_only = psi.empty();
+  const uint64_t N = plan_only ? 0 : 1ull << n;
   auto val = [&](uint32_t off, int k) {
     return C((double)pool[off + 2 * k], (double)pool[off + 2 * k + 1]);
   };
   for (const OpRec &o : ops) {
     const uint32_t code = o.code;
+    if (g_hist && code < 64) ++g_hist[code];
     int kind = 0, j = 0;
     if (code >= CODE_U1 && code <= CODE_U1 + kMaxA) kind = OP_U1, j = code - CODE_U1;
     else if (code >= CODE_C1 && code <= CODE_C1 + kMaxA) kind = OP_C1, j = code - CODE_C1;
@@ -208,10 +212,13 @@ extern "C" {
 // row-major matrix (concatenated in `mats`).  stats[0..3] = sweeps, gates
 // run by the host per-gate path, sweep op records, merged gates.  Returns 0,
 // or 1 with a message in err.
+// psi == NULL: plan only (descriptors are decoded and checked, no state).
+// hist (64 entries, may be NULL) counts the sweep op records per op code.
 int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const uint64_t *targets,
-            const int *ks, uint64_t *stats, char *err, int errlen) {
+            const int *ks, uint64_t *stats, uint64_t *hist, char *err, int errlen) {
   g_err.clear();
-  std::vector<C> st(1ull << n);
+  g_hist = hist;
+  std::vector<C> st(psi ? 1ull << n : 0);
   for (size_t i = 0; i < st.size(); ++i) st[i] = C(psi[2 * i], psi[2 * i + 1]);
   std::vector<Gate> gates;
   size_t moff = 0, toff = 0;
@@ -229,7 +236,7 @@ int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const u
   run.n = n;
   run.dtype_c64 = c64 != 0;
   run.single = [&](const Gate &g) {
-    apply_gate_host(st, n, g);
+    if (!st.empty()) apply_gate_host(st, n, g);
     ++singles;
     return hipSuccess;
   };
@@ -248,7 +255,7 @@ int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const u
     std::snprintf(err, (size_t)errlen, "%s", g_err.empty() ? "planner error" : g_err.c_str());
     return 1;
   }
-  for (size_t i = 0; i < st.size(); ++i) {
+  for (size_t i = 0; psi && i < st.size(); ++i) {
     psi[2 * i] = st[i].real();
     psi[2 * i + 1] = st[i].imag();
   }

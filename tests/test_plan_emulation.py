@@ -44,7 +44,7 @@ def emu(tmp_path_factory):
     lib.emu_run.restype = ctypes.c_int
     lib.emu_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                            ctypes.c_char_p, ctypes.c_int]
+                            ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
 
     def run(n, dtype, psi, opl):
         st = np.ascontiguousarray(psi, dtype=np.complex128).copy()
@@ -57,7 +57,7 @@ def emu(tmp_path_factory):
         err = ctypes.create_string_buffer(512)
         rc = lib.emu_run(n, 1 if dtype == "c64" else 0, st.ctypes.data, len(opl),
                          mats.ctypes.data, tg.ctypes.data, ks.ctypes.data, stats.ctypes.data,
-                         err, 512)
+                         None, err, 512)
         assert rc == 0, err.value.decode()
         return st, dict(zip(["sweeps", "singles", "sweep_ops", "merged"], map(int, stats)))
     return run
