@@ -227,7 +227,10 @@ void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, std::vec
     op.cval = o.pv;
     for (int k = 0; k < 4; ++k) put(o.m0[k]);
     for (int k = 0; k < 4; ++k) put(o.m1[k]);
-    op.code = op_code(op.type, op.s0, 0, (o.pm & reg_qubits) != 0);
+    const int form = mux_form(o.t, o.pm, slot, reg_qubits);
+    if (form == 3) op.s1 = (uint32_t)slot[__builtin_ctzll(o.pm)];
+    op.code = op_code(op.type, op.s0, op.s1,
+                      form == 1 ? MUX_LANES : (form == 2 ? MUX_REGS : MUX_ONE_REG));
   }
   ops.push_back(op);
 }

@@ -73,15 +73,27 @@ enum : uint32_t {
   CODE_MUXL = CODE_DENSE2 + kMaxA * kMaxA,
   // MUX on a register slot whose pattern lies on register slots only: the
   // matrix is chosen per register pair, uniformly over the wave.  + slot
-  CODE_MUXR = CODE_MUXL + kMaxA + 1
+  CODE_MUXR = CODE_MUXL + kMaxA + 1,
+  // MUX on a lane qubit whose pattern is one register slot: + that slot
+  CODE_MUXRL = CODE_MUXR + kMaxA
 };
 
-// `reg_pattern`: the MUX pattern lies on register slots only (else on none).
+// Kernel forms of a multiplexed 2x2 (OP_MUX), by where its pattern lies.
+enum : uint32_t {
+  MUX_NONE = 0,    // no kernel form
+  MUX_LANES = 1,   // pattern on no register slot; target anywhere
+  MUX_REGS = 2,    // pattern on register slots only; target a register slot
+  MUX_ONE_REG = 3  // pattern one register slot (s1); target a lane qubit
+};
+
+// `mux_form`: one of MUX_LANES, MUX_REGS, MUX_ONE_REG for OP_MUX.
 __host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t s1,
-                                            bool reg_pattern = false) {
+                                            uint32_t mux_form = MUX_LANES) {
   const uint32_t j0 = s0 >= kLaneSlot ? (uint32_t)kMaxA : s0;
   switch (type) {
-    case OP_MUX: return (reg_pattern ? CODE_MUXR : CODE_MUXL) + j0;
+    case OP_MUX:
+      if (mux_form == MUX_ONE_REG) return CODE_MUXRL + s1;
+      return (mux_form == MUX_REGS ? CODE_MUXR : CODE_MUXL) + j0;
     case OP_U1: return CODE_U1 + j0;
     case OP_C1: return CODE_C1 + j0;
     case OP_X1: return CODE_X1 + j0;
@@ -390,6 +402,44 @@ __device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST Op
   }
 }
 
+// Multiplexed 2x2 on a lane qubit whose pattern is register slot JC alone:
+// the register pairs with slot bit JC set use one matrix, the others the
+// other, so each lane picks its two row pairs once and every register's
+// choice is fixed at compile time.
+template <int JC, int A, typename R, typename I>
+__device__ __forceinline__ void op_mux_lane_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                                const QS_CONST R *m, const Ctx<I> &c,
+                                                const Regs<A, I> &g) {
+  const int k = (int)op.s0 - (int)kLaneSlot;
+  const bool hi = (c.lane >> k) & 1;
+  const int addr = (c.lane ^ (1 << k)) << 2;
+  // the pattern holds where slot JC's bit equals its value in cval
+  const bool v1 = ((I)op.cval & g.gb[JC]) != 0;
+  const QS_CONST R *mb1 = v1 ? m + 8 : m;  // matrix where the slot bit is 1
+  const QS_CONST R *mb0 = v1 ? m : m + 8;  // where it is 0
+  R sr[2], si[2], pr[2], pi[2];
+  sr[0] = hi ? sld(mb0 + 6) : sld(mb0 + 0);
+  si[0] = hi ? sld(mb0 + 7) : sld(mb0 + 1);
+  pr[0] = hi ? sld(mb0 + 4) : sld(mb0 + 2);
+  pi[0] = hi ? sld(mb0 + 5) : sld(mb0 + 3);
+  sr[1] = hi ? sld(mb1 + 6) : sld(mb1 + 0);
+  si[1] = hi ? sld(mb1 + 7) : sld(mb1 + 1);
+  pr[1] = hi ? sld(mb1 + 4) : sld(mb1 + 2);
+  pi[1] = hi ? sld(mb1 + 5) : sld(mb1 + 3);
+#pragma unroll
+  for (int r = 0; r < (1 << A); ++r) {
+    const int b = (r >> JC) & 1;
+    const R xr = a.re[r], xi = a.im[r];
+    const R yr = xchg(xr, addr), yi = xchg(xi, addr);
+    R ur = 0, ui = 0;
+    cfma(ur, ui, sr[b], si[b], xr, xi);
+    cfma(ur, ui, pr[b], pi[b], yr, yi);
+    a.re[r] = ur;
+    a.im[r] = ui;
+    if ((r % kXchgBatch) == kXchgBatch - 1) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <int J, int A, typename R, typename I>
 __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                           const Ctx<I> &c, const Regs<A, I> &g) {
@@ -560,6 +610,11 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       QS_REG1(CODE_MUXR, 2, (op_mux_reg<2, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXR, 3, (op_mux_reg<3, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXR, 4, (op_mux_reg<4, true, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXRL, 0, (op_mux_lane_reg<0, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXRL, 1, (op_mux_lane_reg<1, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXRL, 2, (op_mux_lane_reg<2, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXRL, 3, (op_mux_lane_reg<3, A>(a, op, m, c, g)))
+      QS_REG1(CODE_MUXRL, 4, (op_mux_lane_reg<4, A>(a, op, m, c, g)))
       default: break;
     }
   }

@@ -118,13 +118,19 @@ inline bool pattern_2x2(const Gate &g, int choice, int &t, uint64_t &pm, uint64_
   return false;
 }
 
-// Whether a multiplexed op on t with pattern pm has a kernel form: the
-// pattern on no register slot, or (t on a register slot) on register slots
-// only.
+// The kernel form of a multiplexed op on t with pattern pm: 1 = the pattern
+// on no register slot; 2 = on register slots only, t on a register slot;
+// 3 = on one register slot alone, t on a lane bit; 0 = none.
+inline int mux_form(int t, uint64_t pm, const int *slot, uint64_t reg_qubits) {
+  if (t < 0 || slot[t] < 0) return 0;
+  if ((pm & reg_qubits) == 0) return 1;
+  if (pm & ~reg_qubits) return 0;
+  if (slot[t] < kFirstLaneSlot) return 2;
+  return __builtin_popcountll(pm) == 1 ? 3 : 0;
+}
+
 inline bool mux_encodable(int t, uint64_t pm, const int *slot, uint64_t reg_qubits) {
-  if (t < 0 || slot[t] < 0) return false;
-  if ((pm & reg_qubits) == 0) return true;
-  return (pm & ~reg_qubits) == 0 && slot[t] < kFirstLaneSlot;
+  return mux_form(t, pm, slot, reg_qubits) != 0;
 }
 
 inline void merge_sweep_ops(std::vector<SweepOp> &items, const int *slot, uint64_t reg_qubits) {

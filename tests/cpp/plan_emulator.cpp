@@ -120,6 +120,7 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
     else if (code >= CODE_DENSE2 && code < CODE_MUXL) kind = OP_DENSE2, j = code - CODE_DENSE2;
     else if (code >= CODE_MUXL && code <= CODE_MUXL + kMaxA) kind = 100, j = code - CODE_MUXL;
     else if (code >= CODE_MUXR && code < CODE_MUXR + kMaxA) kind = 101, j = code - CODE_MUXR;
+    else if (code >= CODE_MUXRL && code < CODE_MUXRL + kMaxA) kind = 102, j = code - CODE_MUXRL;
     else return fail("unknown op code " + std::to_string(code));
     const uint64_t cm = o.cmask;
     const uint64_t cv = o.cval & cm;
@@ -135,6 +136,28 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
     auto need_pool = [&](uint32_t cnt) {
       return (size_t)o.moff + cnt <= pool.size() ? true : fail("pool bounds");
     };
+    if (kind == 102) {
+      // lane target; the pattern is register slot j's bit alone
+      if (j >= A) return fail("pattern slot has no qubit");
+      const int t = slot_qubit(kMaxA, o.s0);
+      if (t < 0) return fail("op slot has no qubit");
+      if (!need_pool(16)) return false;
+      const uint64_t bt = 1ull << t, bj = h.gb[j];
+      C m0[4], m1[4];
+      for (int k = 0; k < 4; ++k) {
+        m0[k] = val(o.moff, k);
+        m1[k] = val(o.moff, 4 + k);
+      }
+      for (uint64_t i = 0; i < N; ++i) {
+        if (i & bt) continue;
+        const bool on = (i & bj) == (o.cval & bj);
+        const C *m = on ? m1 : m0;
+        const C x = psi[i], y = psi[i | bt];
+        psi[i] = m[0] * x + m[1] * y;
+        psi[i | bt] = m[2] * x + m[3] * y;
+      }
+      continue;
+    }
     if (kind == OP_U1 || kind == OP_C1 || kind == OP_X1 || kind == 100 || kind == 101) {
       const int t = slot_qubit(j, o.s0);
       if (t < 0) return fail("op slot has no qubit");
