@@ -230,20 +230,22 @@ __device__ __forceinline__ void cfma(R &ore, R &oim, R mr, R mi, R xr, R xi) {
   oim = fma(mi, xr, oim);
 }
 
-// A descriptor value read on a scalar load.  readfirstlane keeps the value
-// opaque: a per-lane choice between two descriptor values then stays a
-// select of two scalars.  Without it the compiler turns `c ? m[6] : m[0]`
-// into one vector load from a per-lane address, a memory round trip inside
-// the op loop.
+// A descriptor value read on a scalar load and pinned in scalar registers.
+// The empty asm (no instruction) keeps a per-lane choice between two
+// descriptor values a select of two scalars: without it the compiler turns
+// `c ? m[6] : m[0]` into one vector load from a per-lane address (a memory
+// round trip inside the op loop), or into divergent branches around the two
+// scalar loads.  Callers load both values first, then select.
 __device__ __forceinline__ double sld(const QS_CONST double *p) {
-  const long long v = *(const QS_CONST long long *)p;
-  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
-  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  double v = *p;
+  asm volatile("" : "+s"(v));
+  return v;
 }
 
 __device__ __forceinline__ float sld(const QS_CONST float *p) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(*(const QS_CONST int *)p));
+  float v = *p;
+  asm volatile("" : "+s"(v));
+  return v;
 }
 
 // 2x2 on register slot J; kCtl selects the controlled form.
@@ -288,8 +290,11 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
   const int k = (int)op.s0 - (int)kLaneSlot;
   const bool hi = (c.lane >> k) & 1;
   const int addr = (c.lane ^ (1 << k)) << 2;
-  const R sr = hi ? sld(m + 6) : sld(m + 0), si = hi ? sld(m + 7) : sld(m + 1);
-  const R pr = hi ? sld(m + 4) : sld(m + 2), pi = hi ? sld(m + 5) : sld(m + 3);
+  R mv[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) mv[k] = sld(m + k);
+  const R sr = hi ? mv[6] : mv[0], si = hi ? mv[7] : mv[1];
+  const R pr = hi ? mv[4] : mv[2], pi = hi ? mv[5] : mv[3];
   Pred<I> p{};
   if constexpr (kCtl) p = make_pred<A>(op, c, g);
 #pragma unroll
@@ -341,8 +346,10 @@ __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpR
   if constexpr (!kRegPattern) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      lr[k] = p.lane_ok ? sld(m + 8 + 2 * k) : sld(m + 2 * k);
-      li[k] = p.lane_ok ? sld(m + 9 + 2 * k) : sld(m + 2 * k + 1);
+      const R r0 = sld(m + 2 * k), i0 = sld(m + 2 * k + 1);
+      const R r1 = sld(m + 8 + 2 * k), i1 = sld(m + 9 + 2 * k);
+      lr[k] = p.lane_ok ? r1 : r0;
+      li[k] = p.lane_ok ? i1 : i0;
     }
   }
 #pragma unroll
@@ -385,10 +392,13 @@ __device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST Op
   // this lane's output row: the coefficient of its own amplitude (s) and of
   // the partner's (p); selects keep the matrix on scalar loads
   const bool on = p.lane_ok;
-  const R sr = hi ? (on ? sld(m + 14) : sld(m + 6)) : (on ? sld(m + 8) : sld(m + 0));
-  const R si = hi ? (on ? sld(m + 15) : sld(m + 7)) : (on ? sld(m + 9) : sld(m + 1));
-  const R pr = hi ? (on ? sld(m + 12) : sld(m + 4)) : (on ? sld(m + 10) : sld(m + 2));
-  const R pi = hi ? (on ? sld(m + 13) : sld(m + 5)) : (on ? sld(m + 11) : sld(m + 3));
+  R mv[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) mv[k] = sld(m + k);
+  const R sr = hi ? (on ? mv[14] : mv[6]) : (on ? mv[8] : mv[0]);
+  const R si = hi ? (on ? mv[15] : mv[7]) : (on ? mv[9] : mv[1]);
+  const R pr = hi ? (on ? mv[12] : mv[4]) : (on ? mv[10] : mv[2]);
+  const R pi = hi ? (on ? mv[13] : mv[5]) : (on ? mv[11] : mv[3]);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const R xr = a.re[r], xi = a.im[r];
@@ -417,15 +427,21 @@ __device__ __forceinline__ void op_mux_lane_reg(Amp<R, 1 << A> &a, const QS_CONS
   const bool v1 = ((I)op.cval & g.gb[JC]) != 0;
   const QS_CONST R *mb1 = v1 ? m + 8 : m;  // matrix where the slot bit is 1
   const QS_CONST R *mb0 = v1 ? m : m + 8;  // where it is 0
+  R w0[8], w1[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    w0[q] = sld(mb0 + q);
+    w1[q] = sld(mb1 + q);
+  }
   R sr[2], si[2], pr[2], pi[2];
-  sr[0] = hi ? sld(mb0 + 6) : sld(mb0 + 0);
-  si[0] = hi ? sld(mb0 + 7) : sld(mb0 + 1);
-  pr[0] = hi ? sld(mb0 + 4) : sld(mb0 + 2);
-  pi[0] = hi ? sld(mb0 + 5) : sld(mb0 + 3);
-  sr[1] = hi ? sld(mb1 + 6) : sld(mb1 + 0);
-  si[1] = hi ? sld(mb1 + 7) : sld(mb1 + 1);
-  pr[1] = hi ? sld(mb1 + 4) : sld(mb1 + 2);
-  pi[1] = hi ? sld(mb1 + 5) : sld(mb1 + 3);
+  sr[0] = hi ? w0[6] : w0[0];
+  si[0] = hi ? w0[7] : w0[1];
+  pr[0] = hi ? w0[4] : w0[2];
+  pi[0] = hi ? w0[5] : w0[3];
+  sr[1] = hi ? w1[6] : w1[0];
+  si[1] = hi ? w1[7] : w1[1];
+  pr[1] = hi ? w1[4] : w1[2];
+  pi[1] = hi ? w1[5] : w1[3];
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const int b = (r >> JC) & 1;
@@ -675,10 +691,16 @@ __device__ __forceinline__ uint64_t xcd_block(uint32_t xcd_chunk) {
 }
 
 // One wave per tile: load, apply every op, store.
+//
+// Occupancy: complex128 at A = 4 and complex64 at A = 5 fit in 168 VGPRs,
+// three waves per SIMD (measured: 6.40 ms vs 6.67 ms per 30-qubit complex128
+// sweep at two waves); the few values that do not fit are spilled outside the
+// op loop.  The 64-bit index variants (more than 32 qubits) need more
+// registers and keep the compiler's own choice.
 template <typename R, int A, typename I>
-__global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restrict__ psi,
-                                                    const char *__restrict__ blob_g,
-                                                    uint32_t xcd_chunk) {
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(((A <= 4 || sizeof(R) == 4) && sizeof(I) == 4) ? 3 : 1))) void k_sweep(
+    typename SV2<R>::T *__restrict__ psi, const char *__restrict__ blob_g, uint32_t xcd_chunk) {
   constexpr int T = kLaneBits + A;
   constexpr int NA = 1 << A;
   const Blob<R> b = open_blob<R>(blob_g);
@@ -704,12 +726,17 @@ __global__ __launch_bounds__(kThreads) void k_sweep(typename SV2<R>::T *__restri
   }
   run_ops<A>(a, b.ops, b.pool, (int)h.nops, c, g);
   typename SNT<R>::T *dst = reinterpret_cast<typename SNT<R>::T *>(psi);
+  // the store addresses are recomputed from an opaque copy of the tile base:
+  // keeping the 2^A load addresses alive across the op loop costs 2^(A+1)
+  // registers
+  I sbase = c.gbase;
+  asm volatile("" : "+v"(sbase));
 #pragma unroll
   for (int r = 0; r < NA; ++r) {
     typename SNT<R>::T v;
     v.x = a.re[r];
     v.y = a.im[r];
-    __builtin_nontemporal_store(v, dst + (c.gbase | goff<A>(g, r)));
+    __builtin_nontemporal_store(v, dst + (sbase | goff<A>(g, r)));
   }
 }
 
