@@ -43,7 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-CLASSES = ["sweep", "gate1", "gate2", "gatek", "big", "diag", "sparse", "other"]
+CLASSES = ["sweep", "gate1", "gate2", "gatek", "big", "diag", "sparse", "perm"]
 
 from quaru_amd import Register, circuits  # noqa: E402
 from quaru_amd import _lib  # noqa: E402

@@ -194,6 +194,7 @@ enum qs_kernel_class {
     QS_K_BIG = 4,    /* more than 5 dense active qubits */
     QS_K_DIAG = 5,   /* diagonal table */
     QS_K_SPARSE = 6, /* phases on a few basis patterns */
+    QS_K_PERM = 7,   /* permutation with factors on more than 5 qubits (gather) */
     QS_K_COUNT = 8
 };
 

@@ -23,6 +23,8 @@ QS_ERR_ARGUMENT = 16
 QS_ERR_OUT_OF_MEMORY = 17
 QS_ERR_HIP = 18
 QS_ERR_NO_DEVICE = 19
+# kernel classes of qs_timing_read (enum qs_kernel_class)
+QS_K_SWEEP, QS_K_GATE1, QS_K_GATE2, QS_K_GATEK, QS_K_BIG, QS_K_DIAG, QS_K_SPARSE, QS_K_PERM = range(8)
 
 # Every symbol include/quaru_hip.h declares, with (restype, argtypes).
 _P = ctypes.c_void_p

@@ -320,6 +320,8 @@ hipError_t apply_fixed(qs_adjoint *c, void *buf, const qs::Gate &g, const void *
       return qs::launch_gate_diag<R>(buf, ntot, g, const_cast<void *>(d_table), c->stream);
     case qs::Gate::kDense:
       return qs::launch_gate_dense<R>(buf, ntot, g, nullptr, nullptr, c->stream);
+    case qs::Gate::kPerm:  // rejected when the gate is added
+      return hipErrorInvalidValue;
   }
   return hipSuccess;
 }
@@ -535,7 +537,7 @@ int qs_pcircuit_gate(qs_pcircuit *c, const double *matrix, const size_t *targets
   op.fwd = qs::classify_matrix(matrix, t.data(), k, 0, 0);
   op.inv = qs::classify_matrix(inv.data(), t.data(), k, 0, 0);
   if ((op.fwd.kind == qs::Gate::kDense && op.fwd.na > 2) ||
-      (op.fwd.kind == qs::Gate::kDiag && op.fwd.na > 2))
+      (op.fwd.kind == qs::Gate::kDiag && op.fwd.na > 2) || op.fwd.kind == qs::Gate::kPerm)
     return qs::set_error(QS_INVALID_DIMENSIONS, k, k, "unsupported fixed gate");
   c->ops.push_back(std::move(op));
   return QS_OK;

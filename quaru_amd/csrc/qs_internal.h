@@ -28,8 +28,11 @@ struct cd {
 //  kSparse: a diagonal whose entries are 1 except the few listed in
 //           (spat, m): amplitude idx is multiplied by m[e] when
 //           (idx & smask) == spat[e]; smask covers act[] and the controls.
+//  kPerm  : a monomial 2^na x 2^na matrix (one nonzero per row and column):
+//           the amplitude at act pattern r becomes m[r] times the one at act
+//           pattern perm[r], where (idx & cmask) == cval.
 struct Gate {
-  enum Kind : int { kIdentity = 0, kDense = 1, kDiag = 2, kSparse = 3 };
+  enum Kind : int { kIdentity = 0, kDense = 1, kDiag = 2, kSparse = 3, kPerm = 4 };
   Kind kind = kIdentity;
   uint64_t cmask = 0, cval = 0;
   int na = 0;
@@ -37,6 +40,7 @@ struct Gate {
   std::vector<cd> m;
   uint64_t smask = 0;
   std::vector<uint64_t> spat;
+  std::vector<uint32_t> perm;
   // Number of logical gates this entry stands for (apply_all counts n).
   int logical = 1;
 };
@@ -74,6 +78,13 @@ template <typename R>
 hipError_t launch_gate_diag(void *psi, int n, const Gate &g, void *d_mat, hipStream_t s);
 template <typename R>
 hipError_t launch_gate_sparse(void *psi, int n, const Gate &g, hipStream_t s);
+// kPerm: d_tab holds perm[2^na] (uint32) then, at byte offset
+// perm_table_phase_offset(na), the 2^na factors in the register's precision;
+// scratch receives a copy of the state.
+size_t perm_table_phase_offset(int na);
+template <typename R>
+hipError_t launch_gate_perm(void *psi, int n, const Gate &g, const void *d_tab, void *scratch,
+                            hipStream_t s);
 
 // Reductions write FP64 results to device memory `d_out` (caller copies).
 // d_part must hold kReduceBlocks * 2 doubles.

@@ -287,6 +287,31 @@ Gate classify(const double *mat, const size_t *targets, size_t k, uint64_t cmask
         break;
       }
   if (!diag) {
+    // a permutation with factors (one nonzero per row and column) on more
+    // than 5 active qubits is a gather (2 passes over the state) instead of
+    // 2^na complex products per amplitude
+    if (g.na > 5) {
+      std::vector<uint32_t> src(d);
+      std::vector<char> used(d, 0);
+      bool mono = true;
+      for (size_t r = 0; r < d && mono; ++r) {
+        size_t nz = 0;
+        for (size_t c = 0; c < d; ++c)
+          if (!is_zero(red[r * d + c])) {
+            ++nz;
+            src[r] = (uint32_t)c;
+          }
+        mono = nz == 1 && !used[src[r]];
+        if (mono) used[src[r]] = 1;
+      }
+      if (mono) {
+        g.kind = Gate::kPerm;
+        g.perm = std::move(src);
+        g.m.resize(d);
+        for (size_t r = 0; r < d; ++r) g.m[r] = red[r * d + g.perm[r]];
+        return g;
+      }
+    }
     g.kind = Gate::kDense;
     g.m = std::move(red);
     return g;
@@ -348,6 +373,35 @@ hipError_t launch_one(qs_register *r, const Gate &g) {
       if (tmp) dm = tmp;
       ++r->st_kernels;
       e = qs::launch_gate_diag<R>(r->psi, r->n, g, dm, r->stream);
+      if (tmp) {
+        hipError_t e2 = hipStreamSynchronize(r->stream);
+        (void)hipFree(tmp);
+        if (e == hipSuccess) e = e2;
+      }
+      return e;
+    }
+    case Gate::kPerm: {
+      const size_t d = (size_t)1 << g.na;
+      const size_t off = qs::perm_table_phase_offset(g.na);
+      std::vector<char> buf(off + d * 2 * sizeof(T));
+      std::memcpy(buf.data(), g.perm.data(), d * sizeof(uint32_t));
+      T *ph = reinterpret_cast<T *>(buf.data() + off);
+      for (size_t i = 0; i < d; ++i) {
+        ph[2 * i] = (T)g.m[i].re;
+        ph[2 * i + 1] = (T)g.m[i].im;
+      }
+      void *dm = nullptr;
+      void *tmp = nullptr;
+      hipError_t e = buf.size() + 256 <= r->ring_cap ? ring_upload(r, buf.data(), buf.size(), &dm)
+                                                     : big_upload(r, buf.data(), buf.size(), &tmp);
+      if (e != hipSuccess) return e;
+      if (tmp) dm = tmp;
+      if (!r->scratch) {
+        e = hipMalloc(&r->scratch, r->state_bytes());
+        if (e != hipSuccess) return e;
+      }
+      ++r->st_kernels;
+      e = qs::launch_gate_perm<R>(r->psi, r->n, g, dm, r->scratch, r->stream);
       if (tmp) {
         hipError_t e2 = hipStreamSynchronize(r->stream);
         (void)hipFree(tmp);
@@ -423,6 +477,7 @@ int gate_class(const Gate &g) {
       return g.na == 1 ? QS_K_GATE1 : g.na == 2 ? QS_K_GATE2 : g.na <= 5 ? QS_K_GATEK : QS_K_BIG;
     case Gate::kDiag: return QS_K_DIAG;
     case Gate::kSparse: return QS_K_SPARSE;
+    case Gate::kPerm: return QS_K_PERM;
     default: return QS_K_COUNT - 1;
   }
 }

@@ -53,6 +53,7 @@ void gate_masks(const Gate &g, uint64_t &x, uint64_t &z) {
   z = 0;
   switch (g.kind) {
     case Gate::kDense:
+    case Gate::kPerm:
       for (int j = 0; j < g.na; ++j) x |= 1ull << g.act[j];
       z = g.cmask;
       break;
@@ -102,6 +103,7 @@ int pool_count(const Gate &g) {
 double single_amps(const Gate &g, int n) {
   switch (g.kind) {
     case Gate::kDense:
+    case Gate::kPerm:
     case Gate::kDiag: return std::ldexp(1.0, n - popc(g.cmask));
     case Gate::kSparse: return (double)g.spat.size() * std::ldexp(1.0, n - popc(g.smask));
     default: return 0.0;
@@ -564,11 +566,12 @@ int sweep_page_qubit(bool c64) {
 // sweep, so the default is no cap.
 int sweep_high_lanes() { return std::max(0, env_int("QS_SWEEP_HIGH_LANES", kLaneBits)); }
 
-// Measured: with 3 fixed low qubits (128-byte runs) the planner saves ~13%
-// of the sweeps but each sweep is ~17% slower on the 30-qubit circuit, so
-// both dtypes keep 4 (256-byte runs for complex128).
+// Measured on the 30-qubit circuit at three waves per SIMD: complex128 with
+// 3 fixed low qubits (128-byte runs, one more free lane bit) needs 81 sweeps
+// of 6.73 ms instead of 92 of 6.39 ms (+7% gates/s); complex64 keeps 4
+// (128-byte runs; 3 gives 70 sweeps of 4.52 ms instead of 79 of 3.46 ms).
 int sweep_low_bits(bool c64) {
-  const int b = env_int(c64 ? "QS_SWEEP_LOW_C64" : "QS_SWEEP_LOW_C128", 4);
+  const int b = env_int(c64 ? "QS_SWEEP_LOW_C64" : "QS_SWEEP_LOW_C128", c64 ? 4 : 3);
   return std::min(kLaneBits, std::max(0, b));
 }
 

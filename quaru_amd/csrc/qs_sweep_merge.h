@@ -20,6 +20,7 @@ inline void touch_masks(const Gate &g, uint64_t &x, uint64_t &z) {
   z = 0;
   switch (g.kind) {
     case Gate::kDense:
+    case Gate::kPerm:
       for (int j = 0; j < g.na; ++j) x |= 1ull << g.act[j];
       z = g.cmask;
       break;

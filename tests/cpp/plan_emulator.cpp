@@ -65,6 +65,19 @@ void apply_gate_host(std::vector<C> &psi, int n, const Gate &g) {
     for (uint64_t i = 0; i < N; ++i)
       for (size_t e = 0; e < g.spat.size(); ++e)
         if ((i & g.smask) == g.spat[e]) psi[i] *= C(g.m[e].re, g.m[e].im);
+  } else if (g.kind == Gate::kPerm) {
+    const std::vector<C> src = psi;
+    uint64_t am = 0;
+    for (int j = 0; j < g.na; ++j) am |= 1ull << g.act[j];
+    for (uint64_t i = 0; i < N; ++i) {
+      if ((i & g.cmask) != g.cval) continue;
+      uint32_t r = 0;
+      for (int j = 0; j < g.na; ++j) r |= (uint32_t)((i >> g.act[j]) & 1) << j;
+      uint64_t s = i & ~am;
+      for (int j = 0; j < g.na; ++j)
+        if ((g.perm[r] >> j) & 1) s |= 1ull << g.act[j];
+      psi[i] = C(g.m[r].re, g.m[r].im) * src[s];
+    }
   }
 }
 

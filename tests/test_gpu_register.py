@@ -482,3 +482,39 @@ def test_shor_slow_period_finding_register_ops():
         r.apply(op)
     ref = oracle_run(oracle.new([False] * 12), opl)
     assert maxdiff(r.state, ref) < 1e-11
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_permutation_gates_gather(dtype):
+    """Permutations with factors on more than 5 qubits run as a gather from a
+    copy of the state (QS_K_PERM), controlled and uncontrolled, on low and
+    high qubits; compared with the oracle."""
+    import ctypes
+    from quaru_amd import _lib
+    from tests.test_plan_emulation import monomial_gate
+    rng = np.random.default_rng(21)
+    n = 14
+    opl = []
+    for k in (6, 8, 10):
+        t = [int(x) for x in rng.permutation(n)[:k]]
+        opl.append(monomial_gate(rng, t))
+        opl.append(ops.Operation(random_unitary(rng, 2), [t[-1]]))
+        opl.append(ops.to_controlled(monomial_gate(rng, t[1:]), t[0]))
+    opl.append(ops.to_quantum_gate(lambda x: (x * 5) % 128 if x < 128 else x, list(range(6, 14))))
+    psi = random_state(rng, n)
+    ref = oracle_run(psi.copy(), opl)
+    r = Register.new([False] * n, dtype)
+    r.state = psi
+    lib = _lib.load()
+    assert lib.qs_timing_enable(r.handle, 1) == 0
+    for op in opl:
+        r.apply(op)
+    ms = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_uint64 * 8)()
+    assert lib.qs_timing_read(r.handle, ms, cnt, 1) == 0
+    # every gate above except the controlled one on 5 active qubits
+    assert cnt[_lib.QS_K_PERM] == 6
+    if dtype == "c128":
+        assert maxdiff(r.state, ref) < TOL
+    else:
+        assert infid(r.state, ref) < 1e-6

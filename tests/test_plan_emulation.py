@@ -102,3 +102,30 @@ def test_gate_zoo_plans(emu, n):
     zoo = gate_zoo(rng, n) + gate_zoo(rng, n)
     got, _ = emu(n, "c128", psi, zoo)
     assert np.max(np.abs(got - oracle_run(psi.copy(), zoo))) < TOL["c128"]
+
+
+def monomial_gate(rng, targets):
+    """A permutation with unit factors on `targets` (the form to_quantum_gate
+    builds, with phases added)."""
+    d = 1 << len(targets)
+    m = np.zeros((d, d), dtype=np.complex128)
+    m[np.arange(d), rng.permutation(d)] = np.exp(1j * rng.uniform(0, 2 * np.pi, d))
+    return ops.Operation(m, list(targets))
+
+
+def test_permutation_gates_plan(emu):
+    """Permutation gates on more than 5 qubits (the gather form) between
+    sweeps: classified tables and ordering match the oracle."""
+    rng = np.random.default_rng(5)
+    n = 11
+    opl = []
+    for k in (6, 7, 8):
+        t = [int(x) for x in rng.permutation(n)[:k]]
+        opl.append(monomial_gate(rng, t))
+        opl.append(ops.Operation(random_unitary(rng, 2), [t[0]]))
+        opl.append(ops.to_controlled(monomial_gate(rng, t[1:]), t[0]))
+    opl.append(ops.to_quantum_gate(lambda x: (x * 7) % 64 if x < 64 else x, list(range(1, 8))))
+    psi = random_state(rng, n)
+    got, st = emu(n, "c128", psi, opl)
+    assert st["singles"] >= 7
+    assert np.max(np.abs(got - oracle_run(psi.copy(), opl))) < TOL["c128"]
