@@ -667,8 +667,58 @@ class Register {
   }
 
   // ---- circuits (src/register.rs:345-377) -------------------------------
+  // Register::apply_circuit kept on the device: measurements decided on the
+  // GPU (same uniforms, same rule as measure) and every conditional
+  // operation applied as an operation controlled on its condition's measured
+  // qubits.  Exact, because a circuit's conditional operations never act on
+  // measured qubits (src/operation.rs:302-321): after the collapse, "target
+  // measured v" holds exactly where qubit `target` is v.  One
+  // synchronisation at the end reads the outcomes.
   std::vector<std::pair<size_t, bool>> apply_circuit(QuantumCircuit &circuit) {
     for (const Operation &op : circuit.get_operations()) apply(op);
+    const std::vector<size_t> targets = circuit.get_measurement_targets();
+    if (targets.size() > QS_MEASURE_SLOTS) return apply_circuit_host(circuit);
+    for (size_t j = 0; j < targets.size(); ++j)
+      detail::check(qs_measure_deferred(h_, targets[j], rng_.uniform(), j));
+    for (const auto &cop : circuit.get_conditional_operations()) {
+      std::vector<size_t> ctl;
+      std::vector<unsigned char> cv;
+      bool never = false;  // (t, true) and (t, false) cannot both be measured
+      for (const auto &c : cop.second) {
+        size_t k = 0;
+        while (k < ctl.size() && ctl[k] != c.first) ++k;
+        if (k == ctl.size()) {
+          ctl.push_back(c.first);
+          cv.push_back(c.second ? 1 : 0);
+        } else if (cv[k] != (c.second ? 1 : 0)) {
+          never = true;
+        }
+      }
+      if (never) continue;
+      const Operation &op = cop.first;
+      if (op.is_structured()) {
+        for (size_t k = 0; k < op.controls().size(); ++k) {
+          ctl.push_back(op.controls()[k]);
+          cv.push_back(op.control_values().empty() || op.control_values()[k] ? 1 : 0);
+        }
+      }
+      const Matrix &m = op.active_matrix();
+      const int rc = qs_apply_controlled(h_, ctl.data(), cv.data(), ctl.size(),
+                                         reinterpret_cast<const double *>(m.data.data()),
+                                         op.active_targets().data(), op.active_targets().size());
+      if (auto e = detail::status(rc))
+        Applied::err(*e).expect("Could not apply operation");
+    }
+    std::vector<unsigned char> out(targets.size());
+    detail::check(qs_read_slots(h_, 0, targets.size(), out.data()));
+    std::vector<std::pair<size_t, bool>> measures;
+    for (size_t j = 0; j < targets.size(); ++j) measures.emplace_back(targets[j], out[j] != 0);
+    return measures;
+  }
+
+  // The same with the outcomes decided on the host, one measure at a time
+  // (more measurements than device slots).
+  std::vector<std::pair<size_t, bool>> apply_circuit_host(QuantumCircuit &circuit) {
     std::vector<std::pair<size_t, bool>> measures;
     if (!circuit.get_measurement_targets().empty()) measures = measure_circuit(circuit);
     for (const auto &cop : circuit.get_conditional_operations()) {

@@ -126,6 +126,27 @@ int qs_apply_all(qs_register *r, const double *matrix, size_t rows, size_t cols,
  * sqrt(prob).  target >= n -> QS_INVALID_TARGET(target). */
 int qs_measure(qs_register *r, size_t target, double uniform, int *result);
 
+/* Device slots for the outcomes of deferred measurements. */
+#define QS_MEASURE_SLOTS 64
+
+/* qs_measure with the outcome decided on the device and kept in device slot
+ * `slot` (< QS_MEASURE_SLOTS): no host round trip, nothing synchronises.
+ * Same rule and arithmetic as qs_measure (x > prob_0 on the same FP64
+ * sums), so the same uniform gives the same outcome.  Replaces the measure
+ * loop of Register::measure_circuit (src/register.rs:368-375) when
+ * Register::apply_circuit (src/register.rs:345-366) runs on the device: the
+ * circuit's conditional operations never act on measured qubits
+ * (src/operation.rs:302-321), so after the collapse a condition "target
+ * measured v" holds exactly where qubit `target` is v, and each conditional
+ * operation is applied as an operation controlled on those qubits.
+ * target >= n -> QS_INVALID_TARGET(target); slot out of range ->
+ * QS_ERR_ARGUMENT. */
+int qs_measure_deferred(qs_register *r, size_t target, double uniform, size_t slot);
+
+/* Synchronise and copy `count` deferred outcomes (0 or 1) from slot `first`
+ * on.  No reference counterpart: the read-back of qs_measure_deferred. */
+int qs_read_slots(qs_register *r, size_t first, size_t count, unsigned char *out);
+
 /* `count` successive qs_measure calls on order[0..count) with uniforms
  * [0..count) -- the full-register readout of examples/shors.rs:219-222 and
  * examples/grovers.rs:156 -- done in one probability pass.  *index gets bit

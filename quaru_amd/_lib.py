@@ -23,6 +23,7 @@ QS_ERR_ARGUMENT = 16
 QS_ERR_OUT_OF_MEMORY = 17
 QS_ERR_HIP = 18
 QS_ERR_NO_DEVICE = 19
+QS_MEASURE_SLOTS = 64
 # kernel classes of qs_timing_read (enum qs_kernel_class)
 QS_K_SWEEP, QS_K_GATE1, QS_K_GATE2, QS_K_GATEK, QS_K_BIG, QS_K_DIAG, QS_K_SPARSE, QS_K_PERM = range(8)
 
@@ -65,6 +66,8 @@ SIGNATURES = {
     "qs_sync": (_I, [_P]),
     "qs_set_fusion": (_I, [_P, _I]),
     "qs_stats": (_I, [_P, _U64P, _U64P, _U64P, _I]),
+    "qs_measure_deferred": (_I, [_P, _S, _D, _S]),
+    "qs_read_slots": (_I, [_P, _S, _S, _P]),
     "qs_stats_sweep_ops": (_I, [_P, _U64P, _U64P, _I]),
     "qs_timing_enable": (_I, [_P, _I]),
     "qs_timing_read": (_I, [_P, _P, _P, _I]),

@@ -94,6 +94,13 @@ hipError_t launch_prob_split(const void *psi, int n, int target, double *d_part,
                              hipStream_t s);
 template <typename R>
 hipError_t launch_collapse(void *psi, int n, int target, int keep, double total, hipStream_t s);
+// Measure and collapse with the outcome decided on the device (no host
+// round trip): d_probs (2 doubles) and d_coll (2 doubles) are work space, the
+// outcome (0 or 1) goes to *d_slot.
+template <typename R>
+hipError_t launch_measure_deferred(void *psi, int n, int target, double x, double *d_part,
+                                   double *d_probs, double *d_coll, unsigned char *d_slot,
+                                   hipStream_t s);
 template <typename R>
 hipError_t launch_norm(const void *psi, int n, double *d_part, double *d_out, hipStream_t s);
 hipError_t launch_inner(const void *a, bool a_c64, const void *b, bool b_c64, int n,

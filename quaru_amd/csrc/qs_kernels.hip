@@ -744,6 +744,57 @@ __global__ __launch_bounds__(kBlock) void k_collapse(typename V2<R>::T *__restri
   }
 }
 
+// Deferred measurement, decided on the device: probs = {prob_0, prob_1} from
+// k_prob_split, the rule of src/register.rs:317-321 (result = x > prob_0),
+// the outcome into *slot and {keep, sqrt(total)} into coll for
+// k_collapse_dev.
+__global__ void k_measure_decide(const double *__restrict__ probs, double x,
+                                 unsigned char *__restrict__ slot, double *__restrict__ coll) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const double p0 = probs[0], p1 = probs[1];
+  const int res = x > p0;
+  coll[0] = (double)res;
+  coll[1] = sqrt(res ? p1 : p0);
+  *slot = (unsigned char)res;
+}
+
+// k_collapse with its outcome and scale read from device memory.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_collapse_dev(typename V2<R>::T *__restrict__ psi,
+                                                         uint64_t len, int target,
+                                                         const double *__restrict__ coll) {
+  using T = typename V2<R>::T;
+  const int keep = (int)coll[0];
+  const double sq = coll[1];
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * kBlock) {
+    T a = psi[i];
+    if ((int)((i >> target) & 1ull) != keep) {
+      a = czero<T>();
+    } else {
+      a.x = (R)((double)a.x / sq);
+      a.y = (R)((double)a.y / sq);
+    }
+    psi[i] = a;
+  }
+}
+
+template <typename R>
+hipError_t launch_measure_deferred(void *psi, int n, int target, double x, double *d_part,
+                                   double *d_probs, double *d_coll, unsigned char *d_slot,
+                                   hipStream_t s) {
+  hipError_t e = launch_prob_split<R>(psi, n, target, d_part, d_probs, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_measure_decide, dim3(1), dim3(64), 0, s, (const double *)d_probs, x,
+                     d_slot, d_coll);
+  const uint64_t len = 1ull << n;
+  unsigned grid = grid_for(len, kBlock * 4);
+  if (grid > 262144) grid = 262144;
+  hipLaunchKernelGGL(k_collapse_dev<R>, dim3(grid), dim3(kBlock), 0, s, (typename V2<R>::T *)psi,
+                     len, target, (const double *)d_coll);
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_collapse(void *psi, int n, int target, int keep, double total, hipStream_t s) {
   const uint64_t len = 1ull << n;
@@ -1081,6 +1132,8 @@ hipError_t launch_chunk_sums(const void *psi, int n, uint64_t chunk, double *d_o
   template hipError_t launch_prob_split<R>(const void *, int, int, double *, double *,          \
                                            hipStream_t);                                         \
   template hipError_t launch_collapse<R>(void *, int, int, int, double, hipStream_t);           \
+  template hipError_t launch_measure_deferred<R>(void *, int, int, double, double *, double *,  \
+                                                 double *, unsigned char *, hipStream_t);       \
   template hipError_t launch_norm<R>(const void *, int, double *, double *, hipStream_t);       \
   template hipError_t launch_neq_count<R>(const void *, const void *, int, double, double *,    \
                                           double *, hipStream_t);                                \

@@ -585,6 +585,12 @@ int measure_one(qs_register *r, size_t target, double x, int *result) {
   return QS_OK;
 }
 
+// Device slots of deferred measurements: after the reduction results and the
+// collapse parameters in the tail of d_red.
+unsigned char *slot_base(qs_register *r) {
+  return reinterpret_cast<unsigned char *>(r->d_red + 4 * qs::kReduceBlocks + 32);
+}
+
 int measure_dispatch(qs_register *r, size_t target, double x, int *result) {
   if (r->dtype == QS_C128) return measure_one<double>(r, target, x, result);
   return measure_one<float>(r, target, x, result);
@@ -1001,6 +1007,34 @@ int qs_measure(qs_register *r, size_t target, double uniform, int *result) {
   int rc = flush(r);
   if (rc) return rc;
   return measure_dispatch(r, target, uniform, result);
+}
+
+int qs_measure_deferred(qs_register *r, size_t target, double uniform, size_t slot) {
+  if (!r) return fail(QS_ERR_ARGUMENT, 0, 0, "NULL argument");
+  if (target >= (size_t)r->n) return fail(QS_INVALID_TARGET, target, 0, "target out of range");
+  if (slot >= QS_MEASURE_SLOTS) return fail(QS_ERR_ARGUMENT, slot, 0, "slot out of range");
+  int rc = flush(r);
+  if (rc) return rc;
+  double *res_dev = r->d_red + 4 * qs::kReduceBlocks;
+  auto go = [&](auto z) {
+    using Rt = decltype(z);
+    return qs::launch_measure_deferred<Rt>(r->psi, r->n, (int)target, uniform, r->d_red, res_dev,
+                                           res_dev + 2, slot_base(r) + slot, r->stream);
+  };
+  QS_HIP(r->dtype == QS_C128 ? go(0.0) : go(0.0f));
+  return QS_OK;
+}
+
+int qs_read_slots(qs_register *r, size_t first, size_t count, unsigned char *out) {
+  if (!r || (count > 0 && !out)) return fail(QS_ERR_ARGUMENT, 0, 0, "NULL argument");
+  if (first > QS_MEASURE_SLOTS || count > QS_MEASURE_SLOTS - first)
+    return fail(QS_ERR_ARGUMENT, first, count, "slot range out of range");
+  int rc = flush(r);
+  if (rc) return rc;
+  if (count == 0) return QS_OK;
+  QS_HIP(hipMemcpyAsync(out, slot_base(r) + first, count, hipMemcpyDeviceToHost, r->stream));
+  QS_HIP(hipStreamSynchronize(r->stream));
+  return QS_OK;
 }
 
 int qs_measure_sequence(qs_register *r, const size_t *order, size_t count,

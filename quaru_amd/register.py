@@ -310,8 +310,40 @@ class Register:
 
     # -- circuits (src/register.rs:345-377) ------------------------------------
     def apply_circuit(self, circuit: QuantumCircuit) -> List[Tuple[int, bool]]:
+        """``Register::apply_circuit`` (src/register.rs:345-366), kept on the
+        device: the measurements are decided on the GPU (same uniforms, same
+        rule as ``measure``), and each conditional operation is applied as an
+        operation controlled on its condition's measured qubits.  That is
+        exact because a circuit's conditional operations never act on a
+        measured qubit (src/operation.rs:302-321): after the collapse,
+        "target measured v" holds exactly where qubit ``target`` is v.  One
+        synchronisation at the end reads the outcomes."""
         for op in circuit.get_operations():
             self.apply(op)
+        targets = list(circuit.get_measurement_targets())
+        if len(targets) > _lib.QS_MEASURE_SLOTS:
+            return self._apply_circuit_host(circuit)
+        lib = _lib.load()
+        for slot, t in enumerate(targets):
+            _check(lib.qs_measure_deferred(self._h, int(t), self._rng.uniform(), slot))
+        for op, cond in circuit.get_conditional_operations():
+            want = {}
+            for t, v in cond:
+                want.setdefault(int(t), set()).add(bool(v))
+            if any(len(v) > 1 for v in want.values()):
+                continue  # (t, true) and (t, false): never both measured
+            ctl = sorted(want)
+            if not ctl:
+                self.apply(op)
+                continue
+            self.apply_controlled(ctl, op.matrix(), op.targets(),
+                                  [next(iter(want[t])) for t in ctl])
+        out = np.zeros(max(len(targets), 1), dtype=np.uint8)
+        _check(lib.qs_read_slots(self._h, 0, len(targets), _vp(out)))
+        return [(int(t), bool(out[j])) for j, t in enumerate(targets)]
+
+    def _apply_circuit_host(self, circuit: QuantumCircuit) -> List[Tuple[int, bool]]:
+        # more measurements than device slots: measure and decide on the host
         measures: List[Tuple[int, bool]] = []
         if circuit.get_measurement_targets():
             measures = self.measure_circuit(circuit)

@@ -518,3 +518,92 @@ def test_permutation_gates_gather(dtype):
         assert maxdiff(r.state, ref) < TOL
     else:
         assert infid(r.state, ref) < 1e-6
+
+
+def circuit_with_conditions(rng, n):
+    """A QuantumCircuit with operations, measurements of 3 qubits and
+    conditional operations on the others (single, multiple, contradictory
+    and empty conditions)."""
+    c = ops.QuantumCircuit()
+    q = [int(x) for x in rng.permutation(n)]
+    meas, rest = q[:3], q[3:]
+    for _ in range(12):
+        t = [int(x) for x in rng.permutation(n)[:2]]
+        c.add_operation(ops.Operation(random_unitary(rng, 4), t))
+    for t in meas:
+        c.add_measurement(t)
+    c.add_conditional_operation(ops.not_(rest[0]), [(meas[0], True)])
+    c.add_conditional_operation(ops.Operation(random_unitary(rng, 2), [rest[1]]),
+                                [(meas[1], False), (meas[2], True)])
+    c.add_conditional_operation(ops.pauli_z(rest[0]), [(meas[0], True), (meas[0], False)])
+    c.add_conditional_operation(ops.Operation(random_unitary(rng, 4), rest[1:3]), [])
+    c.add_conditional_operation(ops.cnot(rest[2], rest[0]), [(meas[2], False), (meas[2], False)])
+    return c
+
+
+def oracle_circuit(n, circuit, seed):
+    """Register::apply_circuit (src/register.rs:345-366) on the oracle with
+    the register's uniform stream."""
+    rng = circuits.SplitMix64(seed)
+    st = oracle.new([False] * n)
+    for op in circuit.get_operations():
+        st = oracle.apply_fast(st, op.matrix(), op.targets())
+    measures = []
+    for t in circuit.get_measurement_targets():
+        rc, res, st, _ = oracle.try_measure(st, t, rng.uniform())
+        measures.append((t, res))
+    for op, cond in circuit.get_conditional_operations():
+        if all((t, v) in measures for (t, v) in cond):
+            st = oracle.apply_fast(st, op.matrix(), op.targets())
+    return st, measures
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_apply_circuit_on_device_matches_oracle(dtype):
+    """Measurements decided on the device and conditions applied as controls:
+    the same outcomes as the oracle on the same uniforms, and the same state."""
+    rng = np.random.default_rng(8)
+    n = 9
+    outcomes = set()
+    for seed in range(10):
+        c = circuit_with_conditions(rng, n)
+        ref, want = oracle_circuit(n, c, seed)
+        r = Register.new([False] * n, dtype, seed=seed)
+        got = r.apply_circuit(c)
+        assert got == want
+        outcomes.add(tuple(v for _, v in got))
+        if dtype == "c128":
+            assert maxdiff(r.state, ref) < TOL
+        else:
+            assert maxdiff(r.state, ref) < 1e-5
+    assert len(outcomes) >= 4
+
+
+def test_measure_deferred_equals_measure():
+    import ctypes
+    from quaru_amd import EngineError, _lib
+    rng = np.random.default_rng(9)
+    lib = _lib.load()
+    for trial in range(6):
+        psi = random_state(rng, 12)
+        a = Register.new([False] * 12)
+        b = Register.new([False] * 12)
+        a.state = psi
+        b.state = psi
+        us = rng.uniform(0, 1, 12)
+        res = [a.measure(t, float(us[t])) for t in range(12)]
+        for t in range(12):
+            assert lib.qs_measure_deferred(b.handle, t, float(us[t]), 11 - t) == 0
+        out = (ctypes.c_ubyte * 12)()
+        assert lib.qs_read_slots(b.handle, 0, 12, out) == 0
+        assert [bool(out[11 - t]) for t in range(12)] == res
+        assert maxdiff(a.state, b.state) < 1e-15
+    r = Register.new([False] * 4)
+    with pytest.raises(OperationError) as e:
+        from quaru_amd.register import _check
+        _check(lib.qs_measure_deferred(r.handle, 4, 0.5, 0))
+    assert e.value == OperationError.InvalidTarget(4)
+    with pytest.raises(EngineError):
+        _check(lib.qs_measure_deferred(r.handle, 0, 0.5, _lib.QS_MEASURE_SLOTS))
+    with pytest.raises(EngineError):
+        _check(lib.qs_read_slots(r.handle, 60, 5, (ctypes.c_ubyte * 5)()))
