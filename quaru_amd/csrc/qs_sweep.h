@@ -1,21 +1,21 @@
 // qs_sweep.h -- the fused sweep engine.
 //
-// A *sweep* is one pass over the state: each workgroup owns one tile of 2^T
-// amplitudes (the index bits at T chosen qubits, the tile qubits, vary inside
-// a tile; the others are fixed per workgroup), loads it once from HBM, applies
-// a run of gates to it, and stores it once.  Inside the tile every thread
-// keeps 2^A amplitudes in registers (A = T - 8 for 256 threads).  A *phase*
-// names the A tile qubits that are register bits; a dense gate on those
-// qubits is applied in registers with no communication, and moving to the
-// next phase is one exchange through LDS.  Controls and diagonal gates need
-// no data movement at all: every amplitude's global index is known, so they
-// are predicates and multiplies on whichever phase is current.
+// A *sweep* is one pass over the state: each wave owns one tile of 2^T
+// amplitudes (T = 6 + A; the index bits at the T tile qubits vary inside a
+// tile, the others are fixed per wave), loads it once from HBM into
+// registers, applies a run of gates to it, and stores it once.  Every lane
+// keeps 2^A amplitudes in registers: a gate on a register qubit pairs
+// registers inside a lane, a gate on a lane qubit exchanges with the partner
+// lane.  Controls and diagonal gates need no data movement at all: every
+// amplitude's global index is known, so they are predicates and multiplies.
 //
 // The planner packs the pending gate list into sweeps, moving a gate ahead of
 // earlier ones only when it commutes with every gate it passes (on every
 // shared qubit both act diagonally), so the result equals applying the list
-// in order.  Gates the sweep kernel does not handle (dense on 3+ active
-// qubits, large diagonal tables) run as single-gate kernels between sweeps.
+// in order, and merges each sweep's one-qubit gates into multiplexed 2x2 ops
+// (qs_sweep_merge.h).  Gates the sweep kernel does not handle (dense on 3+
+// active qubits, large diagonal tables, permutations) run as single-gate
+// kernels between sweeps.
 #pragma once
 
 #include <functional>

@@ -6,9 +6,9 @@
 // engine beats one pass per gate: the reference's try_apply_fast
 // (src/register.rs:155-189) walks the whole state once per gate.
 //
-// Planner.  A sweep's tile holds qubits 0..3 (lane bits 0..3) plus up to
-// A + 2 more qubits (two lane bits and A register slots).  Gates are taken
-// in order; a later gate may move ahead of earlier, not yet scheduled gates
+// Planner.  A sweep's tile holds qubits 0..b-1 (the fixed low lane bits)
+// plus A + 6 - b more qubits (the other lane bits and A register slots).
+// Gates are taken in order; a later gate may move ahead of earlier, not yet scheduled gates
 // only if it commutes with each of them -- on every shared qubit both act
 // diagonally (a control or a diagonal factor), which makes both operators
 // block-diagonal in that qubit's basis.  A dense gate enters the sweep if

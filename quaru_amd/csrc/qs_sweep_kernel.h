@@ -4,9 +4,9 @@
 // A sweep is one pass over the state.  Each wave (64 lanes) owns a tile of
 // 2^(6+A) amplitudes: every lane keeps 2^A of them in registers.  The tile
 // index bits are assigned as
-//   lane bits 0..b-1 <- qubits 0..b-1 (b = 4 by default: 16 lanes read 16
-//                       consecutive amplitudes, coalesced 256-byte (c128) /
-//                       128-byte (c64) runs),
+//   lane bits 0..b-1 <- qubits 0..b-1 (2^b lanes read 2^b consecutive
+//                       amplitudes: b = 3 for complex128 and b = 4 for
+//                       complex64 by default, 128-byte runs),
 //   lane bits b..5   <- more tile qubits chosen per sweep,
 //   register slots  <- A more tile qubits,
 // and the remaining index bits come from the tile id.  A gate on a register
