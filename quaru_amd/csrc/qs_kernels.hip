@@ -223,7 +223,9 @@ __device__ __forceinline__ uint64_t row_offset(int i, const Fixed &ax) {
 // memory traffic is the state itself.  Consecutive threads take consecutive
 // groups, i.e. consecutive free index bits: with the low qubits free, each
 // of the D loads of a wave is a run of 64 consecutive amplitudes.
-template <typename R, int K>
+// kReal: every matrix entry is real (Hadamard transforms, permutations,
+// RY products), so a complex MAC is 2 FMAs instead of 4.
+template <typename R, int K, bool kReal>
 __global__ __launch_bounds__(kBlock) void k_gatek(typename V2<R>::T *__restrict__ psi,
                                                   uint64_t ngroups, Fixed fx, Fixed ax,
                                                   const R *__restrict__ m_g) {
@@ -246,16 +248,18 @@ __global__ __launch_bounds__(kBlock) void k_gatek(typename V2<R>::T *__restrict_
     R sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
 #pragma unroll
     for (int j = 0; j < D; j += 2) {
-      const R mr0 = row[2 * j], mi0 = row[2 * j + 1];
-      const R mr1 = row[2 * j + 2], mi1 = row[2 * j + 3];
+      const R mr0 = row[2 * j], mr1 = row[2 * j + 2];
       sr0 = fma(mr0, ar[j], sr0);
-      sr0 = fma(-mi0, ai[j], sr0);
       si0 = fma(mr0, ai[j], si0);
-      si0 = fma(mi0, ar[j], si0);
       sr1 = fma(mr1, ar[j + 1], sr1);
-      sr1 = fma(-mi1, ai[j + 1], sr1);
       si1 = fma(mr1, ai[j + 1], si1);
-      si1 = fma(mi1, ar[j + 1], si1);
+      if constexpr (!kReal) {
+        const R mi0 = row[2 * j + 1], mi1 = row[2 * j + 3];
+        sr0 = fma(-mi0, ai[j], sr0);
+        si0 = fma(mi0, ar[j], si0);
+        sr1 = fma(-mi1, ai[j + 1], sr1);
+        si1 = fma(mi1, ar[j + 1], si1);
+      }
     }
     T out;
     out.x = sr0 + sr1;
@@ -287,7 +291,7 @@ __device__ __forceinline__ int stage_phys(int s) { return s ^ ((s >> 5) & 31); }
 // loads over the block's lowest index bits, each lane computes its own group
 // from registers (matrix on scalar loads, as in k_gatek), writes its outputs
 // back to its own slots, and the wave stores the block the way it loaded it.
-template <typename R, int K>
+template <typename R, int K, bool kReal>
 __global__ __launch_bounds__(64) void k_gatek_staged(typename V2<R>::T *__restrict__ psi,
                                                      uint64_t ngroups, Fixed fx, StageMap sm,
                                                      const R *__restrict__ m_g) {
@@ -334,16 +338,18 @@ __global__ __launch_bounds__(64) void k_gatek_staged(typename V2<R>::T *__restri
     R sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
 #pragma unroll
     for (int j = 0; j < D; j += 2) {
-      const R mr0 = row[2 * j], mi0 = row[2 * j + 1];
-      const R mr1 = row[2 * j + 2], mi1 = row[2 * j + 3];
+      const R mr0 = row[2 * j], mr1 = row[2 * j + 2];
       sr0 = fma(mr0, ar[j], sr0);
-      sr0 = fma(-mi0, ai[j], sr0);
       si0 = fma(mr0, ai[j], si0);
-      si0 = fma(mi0, ar[j], si0);
       sr1 = fma(mr1, ar[j + 1], sr1);
-      sr1 = fma(-mi1, ai[j + 1], sr1);
       si1 = fma(mr1, ai[j + 1], si1);
-      si1 = fma(mi1, ar[j + 1], si1);
+      if constexpr (!kReal) {
+        const R mi0 = row[2 * j + 1], mi1 = row[2 * j + 3];
+        sr0 = fma(-mi0, ai[j], sr0);
+        si0 = fma(mi0, ar[j], si0);
+        sr1 = fma(-mi1, ai[j + 1], sr1);
+        si1 = fma(mi1, ar[j + 1], si1);
+      }
     }
     int ip = 0;
 #pragma unroll
@@ -456,6 +462,21 @@ static uint64_t active_mask(const Gate &g) {
   return m;
 }
 
+// Calls f(integral_constant K, integral_constant real) for K = na in 3..5.
+template <typename F>
+static void gatek_dispatch(int na, bool real, F &&f) {
+  using std::integral_constant;
+  if (real) {
+    if (na == 3) f(integral_constant<int, 3>{}, integral_constant<bool, true>{});
+    else if (na == 4) f(integral_constant<int, 4>{}, integral_constant<bool, true>{});
+    else f(integral_constant<int, 5>{}, integral_constant<bool, true>{});
+  } else {
+    if (na == 3) f(integral_constant<int, 3>{}, integral_constant<bool, false>{});
+    else if (na == 4) f(integral_constant<int, 4>{}, integral_constant<bool, false>{});
+    else f(integral_constant<int, 5>{}, integral_constant<bool, false>{});
+  }
+}
+
 // d_mat layout: K in 3..5 -> transposed D x D; K > 5 -> row major D x D;
 // values in precision R, interleaved.  `scratch` (2^n amplitudes) is needed
 // only when K > 5.
@@ -481,29 +502,26 @@ hipError_t launch_gate_dense(void *psi_v, int n, const Gate &g, void *d_mat, voi
   } else if (g.na >= 3 && g.na <= 5) {
     const Fixed ax = active_positions(g);
     const R *m = (const R *)d_mat;  // row-major {re, im}
+    bool real = true;
+    for (const cd &v : g.m) real = real && v.im == 0.0;
     StageMap sm{};
     // fixed low bits (active or control on qubits 0..3) break the direct
     // kernel's 64-amplitude runs: stage through LDS instead
     if ((fx_mask(fx) & 0xFull) && stage_map(fx, ax, n, sm)) {
       const unsigned sgrid = (unsigned)(nfreeN / 64);
-      if (g.na == 3)
-        hipLaunchKernelGGL((k_gatek_staged<R, 3>), dim3(sgrid), dim3(64), 0, s, psi, nfreeN, fx,
-                           sm, m);
-      else if (g.na == 4)
-        hipLaunchKernelGGL((k_gatek_staged<R, 4>), dim3(sgrid), dim3(64), 0, s, psi, nfreeN, fx,
-                           sm, m);
-      else
-        hipLaunchKernelGGL((k_gatek_staged<R, 5>), dim3(sgrid), dim3(64), 0, s, psi, nfreeN, fx,
-                           sm, m);
+      auto staged = [&](auto kk, auto rr) {
+        hipLaunchKernelGGL((k_gatek_staged<R, decltype(kk)::value, decltype(rr)::value>),
+                           dim3(sgrid), dim3(64), 0, s, psi, nfreeN, fx, sm, m);
+      };
+      gatek_dispatch(g.na, real, staged);
       return hipGetLastError();
     }
     const unsigned grid = grid_for(nfreeN, kBlock);
-    if (g.na == 3)
-      hipLaunchKernelGGL((k_gatek<R, 3>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, m);
-    else if (g.na == 4)
-      hipLaunchKernelGGL((k_gatek<R, 4>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, m);
-    else
-      hipLaunchKernelGGL((k_gatek<R, 5>), dim3(grid), dim3(kBlock), 0, s, psi, nfreeN, fx, ax, m);
+    auto direct = [&](auto kk, auto rr) {
+      hipLaunchKernelGGL((k_gatek<R, decltype(kk)::value, decltype(rr)::value>), dim3(grid),
+                         dim3(kBlock), 0, s, psi, nfreeN, fx, ax, m);
+    };
+    gatek_dispatch(g.na, real, direct);
   } else {
     const Fixed ax = active_positions(g);
     hipError_t e = hipMemcpyAsync(scratch, psi, sizeof(T) << n, hipMemcpyDeviceToDevice, s);

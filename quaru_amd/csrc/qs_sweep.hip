@@ -403,7 +403,10 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     // Toffoli target, a controlled 2x2, a phase pattern, another 2x2) is
     // left for a later sweep: nothing else of this sweep touches t after it,
     // and the later sweep, which has t on its tile anyway, merges the two
-    // into one op.
+    // into one op.  Only for patterns of at most 2 qubits: a wide pattern
+    // (a multi-controlled phase over the whole register) has no kernel form
+    // to merge into, and holding back every gate in front of it would starve
+    // the sweeps.
     if (sweep_merge_ops() && pl.gates.size() > 1) {
       std::vector<char> in_sweep_last(64, 0);
       std::vector<int> keep;
@@ -427,7 +430,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
             bool mergeable = plain_2x2(gates[i], mt);
             for (int choice = 0; choice < 8 && !mergeable; ++choice) {
               if (!pattern_2x2(gates[i], choice, tt, pm, pv, mt)) break;
-              mergeable = tt == g.act[0];
+              mergeable = tt == g.act[0] && popc(pm) <= 2;
             }
             if (mergeable) drop[k] = 1;
             break;

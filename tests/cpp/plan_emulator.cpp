@@ -250,8 +250,11 @@ extern "C" {
 // or 1 with a message in err.
 // psi == NULL: plan only (descriptors are decoded and checked, no state).
 // hist (64 entries, may be NULL) counts the sweep op records per op code.
+// cmasks/cvals (may be NULL): controls of each gate beyond its matrix, as
+// qs_apply_controlled passes them to the classifier.
 int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const uint64_t *targets,
-            const int *ks, uint64_t *stats, uint64_t *hist, char *err, int errlen) {
+            const int *ks, const uint64_t *cmasks, const uint64_t *cvals, uint64_t *stats,
+            uint64_t *hist, char *err, int errlen) {
   g_err.clear();
   g_hist = hist;
   std::vector<C> st(psi ? 1ull << n : 0);
@@ -261,7 +264,8 @@ int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const u
   for (int g = 0; g < ngates; ++g) {
     const int k = ks[g];
     std::vector<size_t> t(targets + toff, targets + toff + k);
-    Gate G = qs::classify_matrix(mats + moff, t.data(), (size_t)k, 0, 0);
+    Gate G = qs::classify_matrix(mats + moff, t.data(), (size_t)k, cmasks ? cmasks[g] : 0,
+                                 cvals ? cvals[g] : 0);
     if (G.kind != Gate::kIdentity) gates.push_back(std::move(G));
     moff += (size_t)2 << (2 * k);
     toff += (size_t)k;

@@ -122,20 +122,28 @@ def gate_zoo(rng, n):
     return zoo
 
 
+def random_orthogonal(rng, d):
+    q, r = np.linalg.qr(rng.standard_normal((d, d)))
+    return q * np.sign(np.diag(r))
+
+
+@pytest.mark.parametrize("real", [False, True])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-def test_block_gates_low_and_high_targets(dtype):
+def test_block_gates_low_and_high_targets(dtype, real):
     """Dense 3..5-qubit blocks on low qubits (the LDS-staged kernel), on high
-    qubits (the direct kernel) and under controls on low qubits."""
-    rng = np.random.default_rng(11)
+    qubits (the direct kernel) and under controls on low qubits; complex
+    matrices and real ones (the 2-FMA form)."""
+    rng = np.random.default_rng(11 if not real else 12)
     n = 14
+    mat = (lambda d: random_orthogonal(rng, d)) if real else (lambda d: random_unitary(rng, d))
     opl = []
     for k in (3, 4, 5):
-        opl.append(ops.Operation(random_unitary(rng, 1 << k), list(range(k))))
-        opl.append(ops.Operation(random_unitary(rng, 1 << k), list(range(n - k, n))[::-1]))
+        opl.append(ops.Operation(mat(1 << k), list(range(k))))
+        opl.append(ops.Operation(mat(1 << k), list(range(n - k, n))[::-1]))
         low = [int(x) for x in rng.permutation(4)[:2]]
         rest = [int(x) for x in rng.permutation(np.arange(4, n))[:k - 1]]
-        opl.append(ops.Operation(random_unitary(rng, 1 << k), [low[0]] + rest))
-        opl.append(ops.to_controlled(ops.Operation(random_unitary(rng, 1 << (k - 1)), rest[: k - 1]),
+        opl.append(ops.Operation(mat(1 << k), [low[0]] + rest))
+        opl.append(ops.to_controlled(ops.Operation(mat(1 << (k - 1)), rest[: k - 1]),
                                      low[1]))
     psi0 = random_state(rng, n)
     ref = oracle_run(psi0.copy(), opl)

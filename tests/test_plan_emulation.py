@@ -42,22 +42,39 @@ def emu(tmp_path_factory):
                    check=True, capture_output=True, timeout=600)
     lib = ctypes.CDLL(out)
     lib.emu_run.restype = ctypes.c_int
-    lib.emu_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
-                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                            ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+    lib.emu_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int] + \
+        [ctypes.c_void_p] * 7 + [ctypes.c_char_p, ctypes.c_int]
 
     def run(n, dtype, psi, opl):
-        st = np.ascontiguousarray(psi, dtype=np.complex128).copy()
-        mats = np.ascontiguousarray(np.concatenate(
-            [np.asarray(op.matrix(), dtype=np.complex128).reshape(-1) for op in opl]))
-        tg = np.ascontiguousarray(np.concatenate(
-            [np.asarray(op.targets(), dtype=np.uint64) for op in opl]), dtype=np.uint64)
-        ks = np.ascontiguousarray([len(op.targets()) for op in opl], dtype=np.int32)
+        """psi None: plan only.  Structured (controlled) operations pass their
+        active matrix and controls, as Register.apply does."""
+        st = None if psi is None else np.ascontiguousarray(psi, dtype=np.complex128).copy()
+        mats, tg, ks, cms, cvs = [], [], [], [], []
+        for op in opl:
+            cm = cv = 0
+            if op.is_structured:
+                m, t = op.active_matrix, op.active_targets
+                vals = op.control_values or [True] * len(op.controls)
+                for c, v in zip(op.controls, vals):
+                    cm |= 1 << c
+                    cv |= (1 << c) if v else 0
+            else:
+                m, t = op.matrix(), op.targets()
+            mats.append(np.asarray(m, dtype=np.complex128).reshape(-1))
+            tg.append(np.asarray(t, dtype=np.uint64))
+            ks.append(len(t))
+            cms.append(cm)
+            cvs.append(cv)
+        mats = np.ascontiguousarray(np.concatenate(mats))
+        tg = np.ascontiguousarray(np.concatenate(tg), dtype=np.uint64)
+        ks = np.ascontiguousarray(ks, dtype=np.int32)
+        cms = np.ascontiguousarray(cms, dtype=np.uint64)
+        cvs = np.ascontiguousarray(cvs, dtype=np.uint64)
         stats = np.zeros(4, dtype=np.uint64)
         err = ctypes.create_string_buffer(512)
-        rc = lib.emu_run(n, 1 if dtype == "c64" else 0, st.ctypes.data, len(opl),
-                         mats.ctypes.data, tg.ctypes.data, ks.ctypes.data, stats.ctypes.data,
-                         None, err, 512)
+        rc = lib.emu_run(n, 1 if dtype == "c64" else 0, None if st is None else st.ctypes.data,
+                         len(opl), mats.ctypes.data, tg.ctypes.data, ks.ctypes.data,
+                         cms.ctypes.data, cvs.ctypes.data, stats.ctypes.data, None, err, 512)
         assert rc == 0, err.value.decode()
         return st, dict(zip(["sweeps", "singles", "sweep_ops", "merged"], map(int, stats)))
     return run
@@ -129,3 +146,18 @@ def test_permutation_gates_plan(emu):
     got, st = emu(n, "c128", psi, opl)
     assert st["singles"] >= 7
     assert np.max(np.abs(got - oracle_run(psi.copy(), opl))) < TOL["c128"]
+
+
+def test_plan_sizes_of_the_benchmark_workloads(emu):
+    """Plan-only runs of the 30-qubit random circuit and the 32-qubit Grover
+    workload (no state): the sweep counts the planner reaches, as guards
+    against planning regressions (holding gates back in front of a
+    register-wide phase once took Grover from 100 sweeps to 268)."""
+    w = circuits.random_circuit(30, 20, seed=2025, triples=8, measure=False)
+    _, st = emu(30, "c128", None, w.ops)
+    assert st["sweeps"] <= 85 and st["merged"] >= 550, st
+    _, st = emu(30, "c64", None, w.ops)
+    assert st["sweeps"] <= 82, st
+    _, opl = circuits.grover_operations(32, 12, seed=4, fused=False)
+    _, st = emu(32, "c128", None, opl)
+    assert st["sweeps"] <= 105, st
