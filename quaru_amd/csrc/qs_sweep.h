@@ -70,6 +70,15 @@ int sweep_low_bits(bool c64);
 // touches; at most A + sweep_high_lanes() of them go into one sweep.
 int sweep_page_qubit(bool c64);
 
+// Whether sweeps use workgroup tiles whose 4 waves add two tile qubits
+// (QS_SWEEP_WAVES_C128 / QS_SWEEP_WAVES_C64, default on).
+bool sweep_wave_bits(bool c64);
+
+// How many of a workgroup tile's wave bits take dense gates
+// (QS_SWEEP_DENSE_WAVES, default 1); the others hold qubits that only
+// controls and diagonal gates touch.
+int sweep_dense_wave_bits();
+
 // Whether one-qubit ops of a sweep are merged into multiplexed 2x2 ops
 // (QS_SWEEP_MERGE, default on).
 bool sweep_merge_ops();

@@ -33,7 +33,8 @@ namespace {
 
 using namespace qs::sweep;
 using namespace qs::merge;
-static_assert(kFirstLaneSlot == (int)kLaneSlot, "slot numbering shared with the merge rules");
+static_assert(kFirstLaneSlot == (int)kLaneSlot && kFirstWaveSlot == (int)kWaveSlot,
+              "slot numbering shared with the merge rules");
 
 constexpr int kMaxOps = 512;     // op records per sweep
 constexpr int kWindow = 384;     // look-ahead over unscheduled gates
@@ -122,6 +123,8 @@ struct Plan {
 struct TileShape {
   int A, lowb, hcap;
   uint64_t high;  // qubits whose stride spans a GPU page or more
+  int W;          // wave bits
+  int WD;         // wave bits that may carry dense gates (the rest take fillers)
 };
 
 bool try_place(const Gate &g, const TileShape &ts, Plan &pl) {
@@ -130,7 +133,7 @@ bool try_place(const Gate &g, const TileShape &ts, Plan &pl) {
   uint64_t need = 0;
   for (int j = 0; j < g.na; ++j) need |= 1ull << g.act[j];
   const uint64_t other = pl.other | (need & ~low);
-  if (popc(other) > ts.A + (kLaneBits - ts.lowb)) return false;
+  if (popc(other) > ts.A + (kLaneBits - ts.lowb) + ts.WD) return false;
   if (popc(other & ts.high) > ts.hcap) return false;
   const int A = ts.A;
   uint64_t mr = pl.must_reg;
@@ -237,27 +240,36 @@ void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, std::vec
   ops.push_back(op);
 }
 
-template <typename R, int A>
+template <typename R, int A, int W>
 hipError_t launch_sweep(SweepRunner &run, const void *d_blob, uint64_t ntiles) {
   using V = typename SV2<R>::T;
-  const uint64_t per_block = kThreads / kWave;
+  const uint64_t per_block = W == 0 ? kThreads / kWave : 1;
   const uint64_t grid = (ntiles + per_block - 1) / per_block;
   const uint32_t xcd_chunk = (sweep_xcd_chunks() && grid % 8 == 0) ? (uint32_t)(grid / 8) : 0u;
   if (run.n <= 32)
-    hipLaunchKernelGGL((k_sweep<R, A, uint32_t>), dim3((unsigned)grid), dim3(kThreads), 0,
+    hipLaunchKernelGGL((k_sweep<R, A, uint32_t, W>), dim3((unsigned)grid), dim3(kThreads), 0,
                        run.stream, (V *)run.psi, (const char *)d_blob, xcd_chunk);
   else
-    hipLaunchKernelGGL((k_sweep<R, A, uint64_t>), dim3((unsigned)grid), dim3(kThreads), 0,
+    hipLaunchKernelGGL((k_sweep<R, A, uint64_t, W>), dim3((unsigned)grid), dim3(kThreads), 0,
                        run.stream, (V *)run.psi, (const char *)d_blob, xcd_chunk);
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_sweep_a(int A, SweepRunner &run, const void *d_blob, uint64_t ntiles) {
+hipError_t launch_sweep_a(int A, int W, SweepRunner &run, const void *d_blob, uint64_t ntiles) {
+  if (W == kWaveBits) {
+    switch (A) {
+      case 3: return launch_sweep<R, 3, kWaveBits>(run, d_blob, ntiles);
+      case 4: return launch_sweep<R, 4, kWaveBits>(run, d_blob, ntiles);
+      case 5: return launch_sweep<R, 5, kWaveBits>(run, d_blob, ntiles);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (W != 0) return hipErrorInvalidValue;
   switch (A) {
-    case 3: return launch_sweep<R, 3>(run, d_blob, ntiles);
-    case 4: return launch_sweep<R, 4>(run, d_blob, ntiles);
-    case 5: return launch_sweep<R, 5>(run, d_blob, ntiles);
+    case 3: return launch_sweep<R, 3, 0>(run, d_blob, ntiles);
+    case 4: return launch_sweep<R, 4, 0>(run, d_blob, ntiles);
+    case 5: return launch_sweep<R, 5, 0>(run, d_blob, ntiles);
     default: return hipErrorInvalidValue;
   }
 }
@@ -266,7 +278,10 @@ template <typename R>
 hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t *sweeps) {
   const int n = run.n;
   const int A = sweep_reg_bits(run.dtype_c64);
-  const int T = kLaneBits + A;
+  // the workgroup's wave bits add tile qubits when the register is large
+  // enough for a tile per workgroup
+  const int W = (sweep_wave_bits(run.dtype_c64) && n >= kLaneBits + A + kWaveBits) ? kWaveBits : 0;
+  const int T = kLaneBits + A + W;
   const size_t G = gates.size();
   if (n < T) {
     for (const Gate &g : gates) {
@@ -284,8 +299,14 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
   // of every load instruction.  High qubits go to register slots first and at
   // most `sweep_high_lanes` of them to lane bits.
   const int hq = sweep_page_qubit(run.dtype_c64);
-  const TileShape ts{A, lowb, A + sweep_high_lanes(),
-                     hq >= 64 ? 0ull : (allq & ~((1ull << hq) - 1ull))};
+  // Each dense gate on a wave bit costs an exchange through LDS with four
+  // barriers.  Measured on the 30-qubit complex128 circuit: with both wave
+  // bits open to dense gates, 62 sweeps of 8.1-8.2 ms; with one, 70 sweeps of
+  // 7.2 ms (about the same gates/s, 0.60 instead of 0.53 of HBM peak per
+  // sweep); with none, 81 sweeps of 6.8 ms.
+  const int WD = std::min(W, sweep_dense_wave_bits());
+  const TileShape ts{A, lowb, A + sweep_high_lanes() + WD,
+                     hq >= 64 ? 0ull : (allq & ~((1ull << hq) - 1ull)), W, WD};
   std::vector<uint64_t> gx(G), gz(G);
   for (size_t i = 0; i < G; ++i) gate_masks(gates[i], gx[i], gz[i]);
   std::vector<char> done(G, 0);
@@ -345,7 +366,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     // pseudo-random full sets (seeded by the sweep number, so plans are
     // reproducible) and keeps the best result.
     if (search_iters > 0) {
-      const int cap = A + nfree;
+      const int cap = A + nfree + WD;
       Plan cur, cand;
       auto climb = [&](uint64_t S, Plan &best) {
         for (int it = 0; it < search_iters; ++it) {
@@ -454,8 +475,9 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
       continue;
     }
     // ---- roles: register slots for the most-used qubits (4x4 qubits
-    // first), the rest of the gate qubits on the free lane bits, then fillers
-    std::vector<int> regs, lanes;
+    // first), the rest of the gate qubits on the free lane bits, the least
+    // used on the wave bits (their ops exchange through LDS), then fillers
+    std::vector<int> regs, lanes, waves;
     for (int q = 0; q < n; ++q)
       if ((pl.must_reg >> q) & 1ull) regs.push_back(q);
     std::vector<int> rest;
@@ -467,20 +489,33 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
       if (ha != hb) return ha;
       return pl.cnt[a] > pl.cnt[b];
     });
+    std::vector<int> rest2;
     for (int q : rest) {
       if ((int)regs.size() < A)
         regs.push_back(q);
       else
+        rest2.push_back(q);
+    }
+    std::stable_sort(rest2.begin(), rest2.end(),
+                     [&](int a, int b) { return pl.cnt[a] > pl.cnt[b]; });
+    for (int q : rest2) {
+      if ((int)lanes.size() < nfree)
         lanes.push_back(q);
+      else
+        waves.push_back(q);
     }
     uint64_t P = low | pl.other;
-    for (int q = lowb; q < n && ((int)regs.size() < A || (int)lanes.size() < nfree); ++q) {
+    for (int q = lowb; q < n &&
+                       ((int)regs.size() < A || (int)lanes.size() < nfree || (int)waves.size() < W);
+         ++q) {
       if ((P >> q) & 1ull) continue;
       P |= 1ull << q;
       if ((int)lanes.size() < nfree)
         lanes.push_back(q);
-      else
+      else if ((int)regs.size() < A)
         regs.push_back(q);
+      else
+        waves.push_back(q);
     }
     SweepHeader h{};
     {
@@ -501,6 +536,11 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     for (int j = 0; j < A; ++j) {
       h.gb[j] = 1ull << regs[j];
       slot[regs[j]] = j;
+    }
+    h.nwave = (uint32_t)W;
+    for (int k = 0; k < W; ++k) {
+      h.wb[k] = 1ull << waves[k];
+      slot[waves[k]] = (int)kWaveSlot + k;
     }
     ops.clear();
     pool.clear();
@@ -541,7 +581,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     hipError_t e = run.upload(blob.data(), blob.size(), &d_blob);
     if (e != hipSuccess) return e;
     if (run.mark) run.mark(true);
-    e = launch_sweep_a<R>(A, run, d_blob, h.ntiles);
+    e = launch_sweep_a<R>(A, W, run, d_blob, h.ntiles);
     if (run.mark) run.mark(false);
     if (e != hipSuccess) return e;
     ++*sweeps;
@@ -576,6 +616,22 @@ int sweep_high_lanes() { return std::max(0, env_int("QS_SWEEP_HIGH_LANES", kLane
 int sweep_low_bits(bool c64) {
   const int b = env_int(c64 ? "QS_SWEEP_LOW_C64" : "QS_SWEEP_LOW_C128", c64 ? 4 : 3);
   return std::min(kLaneBits, std::max(0, b));
+}
+
+// Workgroup tiles with wave bits (QS_SWEEP_WAVES_C128 / _C64, default on).
+bool sweep_wave_bits(bool c64) {
+  static const bool on128 = env_int("QS_SWEEP_WAVES_C128", 1) != 0;
+  // complex64 is bound by arithmetic: measured on the 30-qubit circuit, 61
+  // workgroup-tile sweeps of 4.58 ms give the rate of 79 wave-tile sweeps of
+  // 3.51 ms, so complex64 keeps tiles per wave
+  static const bool on64 = env_int("QS_SWEEP_WAVES_C64", 0) != 0;
+  return c64 ? on64 : on128;
+}
+
+// Wave bits open to dense gates (QS_SWEEP_DENSE_WAVES, 0..2, default 1).
+int sweep_dense_wave_bits() {
+  static const int d = std::min(kWaveBits, std::max(0, env_int("QS_SWEEP_DENSE_WAVES", 1)));
+  return d;
 }
 
 // Merging of one-qubit ops inside a sweep (QS_SWEEP_MERGE, default on).

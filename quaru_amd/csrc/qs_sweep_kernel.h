@@ -38,9 +38,14 @@
 namespace qs {
 namespace sweep {
 
-constexpr int kThreads = 256;  // 4 waves = 4 independent tiles per workgroup
+constexpr int kThreads = 256;  // 4 waves per workgroup
 constexpr int kWave = 64;
 constexpr int kLaneBits = 6;
+// Wave bits: with kWaveBits = 2 the 4 waves of a workgroup share one tile of
+// 2^(6+A+2) amplitudes, the wave index supplying two more tile qubits
+// (gates on them exchange through LDS); with 0 each wave has a tile of its
+// own.
+constexpr int kWaveBits = 2;
 constexpr int kMaxA = 5;
 constexpr int kXchgBatch = 4;  // registers exchanged per scheduling group
 
@@ -54,8 +59,10 @@ enum : uint32_t {
   OP_MUX = 7      // 2x2 on one qubit: pool[0..3] where (idx & cmask) != cval, pool[4..7] where ==
 };
 
-// Slot encoding in s0/s1: 0..A-1 register slot, kLaneSlot + k lane bit k.
+// Slot encoding in s0/s1: 0..A-1 register slot, kLaneSlot + k lane bit k,
+// kWaveSlot + k wave bit k.
 constexpr uint32_t kLaneSlot = 8;
+constexpr uint32_t kWaveSlot = 16;
 
 // The kernel's dispatch code: type and register slots in one number.
 // U1/C1/X1: base + slot (slot kMaxA = any lane bit); DENSE2: base + s0 *
@@ -75,7 +82,12 @@ enum : uint32_t {
   // matrix is chosen per register pair, uniformly over the wave.  + slot
   CODE_MUXR = CODE_MUXL + kMaxA + 1,
   // MUX on a lane qubit whose pattern is one register slot: + that slot
-  CODE_MUXRL = CODE_MUXR + kMaxA
+  CODE_MUXRL = CODE_MUXR + kMaxA,
+  // ops on a wave bit (the slot's wave bit is s0 - kWaveSlot)
+  CODE_U1W = CODE_MUXRL + kMaxA,
+  CODE_C1W = CODE_U1W + 1,
+  CODE_X1W = CODE_C1W + 1,
+  CODE_MUXW = CODE_X1W + 1  // pattern on no register slot
 };
 
 // Kernel forms of a multiplexed 2x2 (OP_MUX), by where its pattern lies.
@@ -90,6 +102,15 @@ enum : uint32_t {
 __host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t s1,
                                             uint32_t mux_form = MUX_LANES) {
   const uint32_t j0 = s0 >= kLaneSlot ? (uint32_t)kMaxA : s0;
+  if (s0 >= kWaveSlot) {
+    switch (type) {
+      case OP_U1: return CODE_U1W;
+      case OP_C1: return CODE_C1W;
+      case OP_X1: return CODE_X1W;
+      case OP_MUX: return mux_form == MUX_LANES ? CODE_MUXW : CODE_NONE;
+      default: return CODE_NONE;
+    }
+  }
   switch (type) {
     case OP_MUX:
       if (mux_form == MUX_ONE_REG) return CODE_MUXRL + s1;
@@ -120,6 +141,8 @@ struct SweepHeader {
   uint64_t ntiles;
   uint64_t gb[8];     // global bit of register slot j
   uint64_t lg[8];     // global bit of lane bit i (i < 6)
+  uint64_t wb[2];     // global bit of wave bit k (k < nwave)
+  uint32_t nwave, pad;
 };
 
 template <typename R>
@@ -175,8 +198,9 @@ __device__ __forceinline__ I goff(const Regs<A, I> &g, int r) {
 
 template <typename I>
 struct Ctx {
-  I gbase;  // tile base | this lane's lane-bit offsets (no register bits)
+  I gbase;  // tile base | this lane's lane-bit and wave-bit offsets (no register bits)
   int lane;
+  int wave;  // wave index in the workgroup
 };
 
 // on(r) = lane_ok && ((goff(r) & cmr) == cvr): the second factor is uniform.
@@ -456,6 +480,101 @@ __device__ __forceinline__ void op_mux_lane_reg(Amp<R, 1 << A> &a, const QS_CONS
   }
 }
 
+// LDS exchange buffer of the wave-bit ops: two regions of 4 waves x NR
+// registers x {re, im} x 64 lanes, one region per exchange round, alternating.
+template <typename R, int NR>
+__device__ __forceinline__ R *wave_xbuf() {
+  __shared__ R buf[2 * 4 * NR * 2 * kWave];
+  return buf;
+}
+
+// Registers exchanged per round: 4 complex128 or 8 complex64 values per lane
+// (32 KiB of LDS per workgroup), and at least two rounds per op.
+template <typename R, int NA>
+constexpr int wave_round_regs() {
+  constexpr int cap = sizeof(R) == 8 ? 4 : 8;
+  return cap < NA / 2 ? cap : NA / 2;
+}
+
+// An op on wave bit k (slot kWaveSlot + k): the partner amplitude sits in
+// the same lane and register of wave w ^ (1 << k).  Every wave writes a round
+// of its registers to LDS, the workgroup synchronises, and every wave reads
+// its partner's values and computes its own outputs, like a lane op.  The
+// rounds alternate between two LDS regions and their count is even, so one
+// barrier per round suffices: a region is rewritten only after the barrier
+// that every wave reaches after reading it.  All waves of a workgroup run the
+// same op list, so they meet the same barriers.
+// KIND: 0 uncontrolled 2x2, 1 controlled 2x2, 2 (controlled) exchange,
+// 3 multiplexed 2x2 whose pattern has no register bit.
+template <int KIND, int A, typename R, typename I>
+__device__ __forceinline__ void op_wave(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                        const QS_CONST R *m, const Ctx<I> &c,
+                                        const Regs<A, I> &g) {
+  constexpr int NA = 1 << A;
+  constexpr int NR = wave_round_regs<R, NA>();
+  static_assert((NA / NR) % 2 == 0, "an even number of exchange rounds");
+  R *xb = wave_xbuf<R, NR>();
+  const int k = (int)op.s0 - (int)kWaveSlot;
+  const int w = c.wave;
+  const bool hi = (w >> k) & 1;  // the same for the whole wave
+  const int pw = w ^ (1 << k);
+  Pred<I> pd{};
+  if constexpr (KIND != 0) pd = make_pred<A>(op, c, g);
+  R sr = 0, si = 0, pr = 0, pi = 0;  // coefficients of this wave's own and partner amplitude
+  if constexpr (KIND == 0 || KIND == 1) {
+    R mv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) mv[q] = sld(m + q);
+    sr = hi ? mv[6] : mv[0];
+    si = hi ? mv[7] : mv[1];
+    pr = hi ? mv[4] : mv[2];
+    pi = hi ? mv[5] : mv[3];
+  } else if constexpr (KIND == 3) {
+    R mv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mv[q] = sld(m + q);
+    const bool on = pd.lane_ok;
+    sr = hi ? (on ? mv[14] : mv[6]) : (on ? mv[8] : mv[0]);
+    si = hi ? (on ? mv[15] : mv[7]) : (on ? mv[9] : mv[1]);
+    pr = hi ? (on ? mv[12] : mv[4]) : (on ? mv[10] : mv[2]);
+    pi = hi ? (on ? mv[13] : mv[5]) : (on ? mv[11] : mv[3]);
+  }
+#pragma unroll
+  for (int r0 = 0; r0 < NA; r0 += NR) {
+    R *reg = xb + ((r0 / NR) & 1) * (4 * NR * 2 * kWave);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      reg[((w * NR + j) * 2 + 0) * kWave + c.lane] = a.re[r0 + j];
+      reg[((w * NR + j) * 2 + 1) * kWave + c.lane] = a.im[r0 + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const int r = r0 + j;
+      const R yr = reg[((pw * NR + j) * 2 + 0) * kWave + c.lane];
+      const R yi = reg[((pw * NR + j) * 2 + 1) * kWave + c.lane];
+      const R xr = a.re[r], xi = a.im[r];
+      if constexpr (KIND == 2) {
+        const bool on = pred_on<A>(pd, g, r);
+        a.re[r] = on ? yr : xr;
+        a.im[r] = on ? yi : xi;
+      } else {
+        R ur = 0, ui = 0;
+        cfma(ur, ui, sr, si, xr, xi);
+        cfma(ur, ui, pr, pi, yr, yi);
+        if constexpr (KIND == 1) {
+          const bool on = pred_on<A>(pd, g, r);
+          a.re[r] = on ? ur : xr;
+          a.im[r] = on ? ui : xi;
+        } else {
+          a.re[r] = ur;
+          a.im[r] = ui;
+        }
+      }
+    }
+  }
+}
+
 template <int J, int A, typename R, typename I>
 __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                           const Ctx<I> &c, const Regs<A, I> &g) {
@@ -572,7 +691,7 @@ __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec 
 // together by the planner, see op_code) rather than nested switches on type
 // and slot: each nesting level is a join where the compiler copied the whole
 // amplitude array between register sets, three whole-array copies per op.
-template <int A, typename R, typename I>
+template <int A, int W, typename R, typename I>
 __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec *ops,
                                         const QS_CONST R *pool, int nops, const Ctx<I> &c,
                                         const Regs<A, I> &g) {
@@ -631,6 +750,18 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       QS_REG1(CODE_MUXRL, 2, (op_mux_lane_reg<2, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXRL, 3, (op_mux_lane_reg<3, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXRL, 4, (op_mux_lane_reg<4, A>(a, op, m, c, g)))
+      case CODE_U1W:
+        if constexpr (W > 0) op_wave<0, A>(a, op, m, c, g);
+        break;
+      case CODE_C1W:
+        if constexpr (W > 0) op_wave<1, A>(a, op, m, c, g);
+        break;
+      case CODE_X1W:
+        if constexpr (W > 0) op_wave<2, A>(a, op, m, c, g);
+        break;
+      case CODE_MUXW:
+        if constexpr (W > 0) op_wave<3, A>(a, op, m, c, g);
+        break;
       default: break;
     }
   }
@@ -690,27 +821,40 @@ __device__ __forceinline__ uint64_t xcd_block(uint32_t xcd_chunk) {
   return (uint64_t)(w & 7u) * xcd_chunk + (w >> 3);
 }
 
-// One wave per tile: load, apply every op, store.
+// This wave's offsets from the workgroup's wave bits.
+template <int W, typename I>
+__device__ __forceinline__ I wave_offset(const QS_CONST SweepHeader &h, int wave) {
+  I wo = 0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) wo |= ((wave >> k) & 1) ? (I)h.wb[k] : (I)0;
+  return wo;
+}
+
+// One tile per wave (W = 0) or per workgroup (W = kWaveBits): load, apply
+// every op, store.
 //
 // Occupancy: complex128 at A = 4 and complex64 at A = 5 fit in 168 VGPRs,
 // three waves per SIMD (measured: 6.40 ms vs 6.67 ms per 30-qubit complex128
 // sweep at two waves); the few values that do not fit are spilled outside the
 // op loop.  The 64-bit index variants (more than 32 qubits) need more
 // registers and keep the compiler's own choice.
-template <typename R, int A, typename I>
+template <typename R, int A, typename I, int W>
 __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(((A <= 4 || sizeof(R) == 4) && sizeof(I) == 4) ? 3 : 1))) void k_sweep(
     typename SV2<R>::T *__restrict__ psi, const char *__restrict__ blob_g, uint32_t xcd_chunk) {
-  constexpr int T = kLaneBits + A;
+  constexpr int T = kLaneBits + A + W;
   constexpr int NA = 1 << A;
   const Blob<R> b = open_blob<R>(blob_g);
   const QS_CONST SweepHeader &h = *b.h;
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t tile = xcd_block(xcd_chunk) * (kThreads / kWave) + (threadIdx.x / kWave);
-  if (tile >= h.ntiles) return;  // whole waves only
+  const int wave = threadIdx.x / kWave;
+  const uint64_t tile =
+      W == 0 ? xcd_block(xcd_chunk) * (kThreads / kWave) + wave : xcd_block(xcd_chunk);
+  if (tile >= h.ntiles) return;  // whole waves (W = 0) or whole workgroups
   const Regs<A, I> g = reg_bits<A, I>(h);
   Ctx<I> c;
-  c.gbase = (I)tile_base<T>(tile, h) | lane_offset<I>(h, lane);
+  c.gbase = (I)tile_base<T>(tile, h) | lane_offset<I>(h, lane) | wave_offset<W, I>(h, wave);
+  c.wave = wave;
   c.lane = lane;
   Amp<R, NA> a;
   {
@@ -724,7 +868,7 @@ __attribute__((amdgpu_waves_per_eu(((A <= 4 || sizeof(R) == 4) && sizeof(I) == 4
       a.im[r] = v[r].y;
     }
   }
-  run_ops<A>(a, b.ops, b.pool, (int)h.nops, c, g);
+  run_ops<A, W>(a, b.ops, b.pool, (int)h.nops, c, g);
   typename SNT<R>::T *dst = reinterpret_cast<typename SNT<R>::T *>(psi);
   // the store addresses are recomputed from an opaque copy of the tile base:
   // keeping the 2^A load addresses alive across the op loop costs 2^(A+1)

@@ -11,8 +11,10 @@
 namespace qs {
 namespace merge {
 
-// Register slots are 0..kLaneSlot-1 in the slot table; lane bits kLaneSlot + k.
+// Register slots are 0..kLaneSlot-1 in the slot table; lane bits kLaneSlot + k,
+// wave bits kWaveSlot + k.
 constexpr int kFirstLaneSlot = 8;
+constexpr int kFirstWaveSlot = 16;
 
 // Qubits a gate touches non-diagonally (x) and diagonally / as control (z).
 inline void touch_masks(const Gate &g, uint64_t &x, uint64_t &z) {
@@ -120,13 +122,14 @@ inline bool pattern_2x2(const Gate &g, int choice, int &t, uint64_t &pm, uint64_
 }
 
 // The kernel form of a multiplexed op on t with pattern pm: 1 = the pattern
-// on no register slot; 2 = on register slots only, t on a register slot;
-// 3 = on one register slot alone, t on a lane bit; 0 = none.
+// on no register slot (t anywhere); 2 = on register slots only, t on a
+// register slot; 3 = on one register slot alone, t on a lane bit; 0 = none.
 inline int mux_form(int t, uint64_t pm, const int *slot, uint64_t reg_qubits) {
   if (t < 0 || slot[t] < 0) return 0;
   if ((pm & reg_qubits) == 0) return 1;
   if (pm & ~reg_qubits) return 0;
   if (slot[t] < kFirstLaneSlot) return 2;
+  if (slot[t] >= kFirstWaveSlot) return 0;  // wave bits: the first form only
   return __builtin_popcountll(pm) == 1 ? 3 : 0;
 }
 

@@ -98,7 +98,9 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
   std::memcpy(ops.data(), (const char *)blob + hb, h.nops * sizeof(OpRec));
   std::vector<R> pool(h.npool);
   std::memcpy(pool.data(), (const char *)blob + hb + h.nops * sizeof(OpRec), h.npool * sizeof(R));
-  const int T = kLaneBits + A;
+  if (h.nwave != 0 && h.nwave != (uint32_t)kWaveBits) return fail("wave bit count");
+  const int W = (int)h.nwave;
+  const int T = kLaneBits + A + W;
   uint64_t tmask = 0, smask = 0, regmask = 0;
   for (int i = 0; i < T; ++i) {
     if (i > 0 && h.tpos[i] <= h.tpos[i - 1]) return fail("tile qubits not ascending");
@@ -113,6 +115,10 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
     if (__builtin_popcountll(h.gb[j]) != 1) return fail("register bit table");
     smask |= h.gb[j];
     regmask |= h.gb[j];
+  }
+  for (int k = 0; k < W; ++k) {
+    if (__builtin_popcountll(h.wb[k]) != 1) return fail("wave bit table");
+    smask |= h.wb[k];
   }
   if (smask != tmask || __builtin_popcountll(tmask) != T) return fail("tile tables disagree");
   if (h.ntiles != (1ull << (n - T))) return fail("tile count");
@@ -134,10 +140,19 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
     else if (code >= CODE_MUXL && code <= CODE_MUXL + kMaxA) kind = 100, j = code - CODE_MUXL;
     else if (code >= CODE_MUXR && code < CODE_MUXR + kMaxA) kind = 101, j = code - CODE_MUXR;
     else if (code >= CODE_MUXRL && code < CODE_MUXRL + kMaxA) kind = 102, j = code - CODE_MUXRL;
+    else if (code == CODE_U1W) kind = OP_U1, j = -1;
+    else if (code == CODE_C1W) kind = OP_C1, j = -1;
+    else if (code == CODE_X1W) kind = OP_X1, j = -1;
+    else if (code == CODE_MUXW) kind = 100, j = -1;
     else return fail("unknown op code " + std::to_string(code));
     const uint64_t cm = o.cmask;
     const uint64_t cv = o.cval & cm;
     auto slot_qubit = [&](int jj, uint32_t s0) -> int {
+      if (jj < 0) {  // wave bit
+        const int k = (int)s0 - (int)kWaveSlot;
+        if (k < 0 || k >= W) return -3;
+        return __builtin_ctzll(h.wb[k]);
+      }
       if (jj < kMaxA) {
         if (jj >= A) return -1;  // the kernel's switch skips such codes
         return __builtin_ctzll(h.gb[jj]);
