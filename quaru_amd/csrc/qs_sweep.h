@@ -57,13 +57,14 @@ struct SweepRunner {
 hipError_t sweep_execute(SweepEngine *e, SweepRunner &run, const std::vector<Gate> &gates,
                          uint64_t *sweeps, uint64_t *kernels);
 
-// Register bits per lane used for a dtype (a tile is 2^(6 + bits) amplitudes).
+// Register bits per lane used for a dtype (a wave holds 2^(6 + bits)
+// amplitudes of its tile).
 int sweep_reg_bits(bool c64);
 
 // Qubits 0..b-1 sit on lane bits 0..b-1 of every tile, so that each group of
-// 2^b lanes reads 2^b consecutive amplitudes (b = 4: 256-byte runs for
-// complex128, 128-byte runs for complex64); the other 6 - b lane bits take
-// qubits the planner chooses per sweep.
+// 2^b lanes reads 2^b consecutive amplitudes (defaults b = 3 for complex128,
+// 4 for complex64: 128-byte runs); the other 6 - b lane bits take qubits the
+// planner chooses per sweep.
 int sweep_low_bits(bool c64);
 
 // Tile qubits at or above sweep_page_qubit() each double the pages a wave
@@ -71,7 +72,7 @@ int sweep_low_bits(bool c64);
 int sweep_page_qubit(bool c64);
 
 // Whether sweeps use workgroup tiles whose 4 waves add two tile qubits
-// (QS_SWEEP_WAVES_C128 / QS_SWEEP_WAVES_C64, default on).
+// (QS_SWEEP_WAVES_C128 / QS_SWEEP_WAVES_C64, defaults on / off).
 bool sweep_wave_bits(bool c64);
 
 // How many of a workgroup tile's wave bits take dense gates

@@ -1,14 +1,16 @@
 // qs_sweep_kernel.h -- device side of the fused sweep engine: the sweep
 // descriptor layout and the sweep kernel.  Included by qs_sweep.hip only.
 //
-// A sweep is one pass over the state.  Each wave (64 lanes) owns a tile of
-// 2^(6+A) amplitudes: every lane keeps 2^A of them in registers.  The tile
-// index bits are assigned as
+// A sweep is one pass over the state.  A tile is owned by one wave (64
+// lanes, 2^(6+A) amplitudes) or by the 4 waves of a workgroup (2^(8+A)
+// amplitudes, the wave index supplying two wave bits); every lane keeps 2^A
+// amplitudes in registers.  The tile index bits are assigned as
 //   lane bits 0..b-1 <- qubits 0..b-1 (2^b lanes read 2^b consecutive
 //                       amplitudes: b = 3 for complex128 and b = 4 for
 //                       complex64 by default, 128-byte runs),
 //   lane bits b..5   <- more tile qubits chosen per sweep,
 //   register slots  <- A more tile qubits,
+//   wave bits        <- 0 or 2 more tile qubits,
 // and the remaining index bits come from the tile id.  A gate on a register
 // qubit pairs registers inside a lane; a gate on a lane qubit fetches the
 // partner lane's amplitude with ds_bpermute (one code path for every lane
