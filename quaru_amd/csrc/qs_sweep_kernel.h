@@ -14,8 +14,9 @@
 // and the remaining index bits come from the tile id.  A gate on a register
 // qubit pairs registers inside a lane; a gate on a lane qubit fetches the
 // partner lane's amplitude with ds_bpermute (one code path for every lane
-// qubit) and each lane computes its own output.  There is no LDS tile and no
-// barrier: a sweep is load, ops, store.
+// qubit) and each lane computes its own output; a gate on a wave bit
+// exchanges with the partner wave through LDS (op_wave).  The tile itself
+// never goes through LDS: a sweep is load, ops, store.
 //
 // Controls are split into the part on register slots, which is the same for
 // the whole wave (scalar arithmetic), and the part on every other index bit,
