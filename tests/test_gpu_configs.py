@@ -66,6 +66,56 @@ def test_config2_26q_mirror_returns_to_zero():
     assert r.equals(_zero(n), TOL)
 
 
+def _streaming_zoo(rng, n):
+    """Per-gate kernels at a size where they stream (state above 256 MiB):
+    one-qubit gates with partners inside a 128-byte line (qubits 0..2) and
+    beyond it, two-qubit gates, 3..5-qubit blocks on low qubits (LDS-staged)
+    and high ones (direct), complex and real, controlled gates, a diagonal
+    and a sparse phase."""
+    from tests.conftest import random_unitary
+
+    def orth(d):
+        q, r = np.linalg.qr(rng.standard_normal((d, d)))
+        return (q * np.sign(np.diag(r))).astype(np.complex128)
+
+    opl = [ops.Operation(random_unitary(rng, 2), [q]) for q in (0, 1, 2, 3, 12, n - 1)]
+    opl += [ops.Operation(random_unitary(rng, 4), [1, n - 2]),
+            ops.Operation(random_unitary(rng, 4), [7, 20])]
+    for k in (3, 4, 5):
+        opl += [ops.Operation(random_unitary(rng, 1 << k), list(range(k))),
+                ops.Operation(random_unitary(rng, 1 << k), list(range(n - k, n))),
+                ops.Operation(orth(1 << k), list(range(8, 8 + k)))]
+    opl += [ops.cnot(0, n - 1), ops.cnx([3, 9], 17), ops.cnz([2, 5, 11], 23), ops.rz(0.7, 14),
+            ops.Operation(np.diag(np.exp(1j * rng.uniform(0, 6, 8))), [4, 13, 22]),
+            ops.to_controlled(ops.Operation(random_unitary(rng, 2), [n - 3]), 1)]
+    return opl
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_streaming_per_gate_kernels_26q(dtype):
+    """Fusion off at 26 qubits (1 GiB complex128, 512 MiB complex64): the
+    per-gate kernels take their nontemporal form; every gate class against
+    the multi-core oracle (1e-12 per amplitude, infidelity <= 1e-6 for
+    complex64)."""
+    n = 26
+    rng = np.random.default_rng(2626)
+    opl = _streaming_zoo(rng, n)
+    r = Register.new([False] * n, dtype).set_fusion(False)
+    r.apply_all(ops.hadamard(0))
+    for op in opl:
+        r.apply(op)
+    ref = np.full(1 << n, 2.0 ** (-n / 2), dtype=np.complex128)
+    scratch = np.empty_like(ref)
+    for op in opl:
+        oracle.apply_fast_omp(ref, scratch, op.matrix(), op.targets(), _threads())
+    got = r.state
+    if dtype == "c128":
+        assert float(np.max(np.abs(got - ref))) < TOL
+    else:
+        inf = 1.0 - abs(np.vdot(ref, got)) ** 2 / (np.vdot(got, got).real * np.vdot(ref, ref).real)
+        assert inf <= 1e-6
+
+
 def test_config3_30q_c64_vs_c128_and_mirror():
     n = 30
     w = circuits.random_circuit(n, 20, seed=2025, triples=8, measure=False)
