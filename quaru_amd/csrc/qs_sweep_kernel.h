@@ -13,8 +13,8 @@
 //   wave bits        <- 0 or 2 more tile qubits,
 // and the remaining index bits come from the tile id.  A gate on a register
 // qubit pairs registers inside a lane; a gate on a lane qubit fetches the
-// partner lane's amplitude with ds_bpermute (one code path for every lane
-// qubit) and each lane computes its own output; a gate on a wave bit
+// partner lane's amplitude with VALU cross-lane moves (DPP, permlane swaps;
+// lane_xor) and each lane computes its own output; a gate on a wave bit
 // exchanges with the partner wave through LDS (op_wave).  The tile itself
 // never goes through LDS: a sweep is load, ops, store.
 //
@@ -34,6 +34,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "qs_internal.h"
 
 #define QS_CONST __attribute__((address_space(4)))
@@ -50,7 +52,6 @@ constexpr int kLaneBits = 6;
 // own.
 constexpr int kWaveBits = 2;
 constexpr int kMaxA = 5;
-constexpr int kXchgBatch = 4;  // registers exchanged per scheduling group
 
 enum : uint32_t {
   OP_U1 = 1,      // uncontrolled 2x2 on one qubit (register slot or lane bit)
@@ -67,27 +68,30 @@ enum : uint32_t {
 constexpr uint32_t kLaneSlot = 8;
 constexpr uint32_t kWaveSlot = 16;
 
-// The kernel's dispatch code: type and register slots in one number.
-// U1/C1/X1: base + slot (slot kMaxA = any lane bit); DENSE2: base + s0 *
-// kMaxA + s1.
+// The kernel's dispatch code: type, register slots and lane bit in one
+// number, so every case runs with its slots and lane bit known at compile
+// time.  U1/C1/X1/MUXL: base + slot j (j < kMaxA) or base + kMaxA + lane
+// bit; DENSE2: base + s0 * kMaxA + s1.
+constexpr uint32_t kLaneCodes = kMaxA + kLaneBits;
 enum : uint32_t {
   CODE_NONE = 0,
-  CODE_U1 = 1,                    // 1 .. 6
-  CODE_C1 = CODE_U1 + kMaxA + 1,  // 7 .. 12
-  CODE_X1 = CODE_C1 + kMaxA + 1,  // 13 .. 18
-  CODE_SPARSE = CODE_X1 + kMaxA + 1,
+  CODE_U1 = 1,                      // 1 .. 11
+  CODE_C1 = CODE_U1 + kLaneCodes,   // 12 .. 22
+  CODE_X1 = CODE_C1 + kLaneCodes,   // 23 .. 33
+  CODE_SPARSE = CODE_X1 + kLaneCodes,
   CODE_DIAG = CODE_SPARSE + 1,
-  CODE_DENSE2 = CODE_DIAG + 1,    // + s0 * kMaxA + s1
+  CODE_DENSE2 = CODE_DIAG + 1,      // + s0 * kMaxA + s1
   // MUX whose pattern has no register-slot bit: the matrix is chosen once
-  // per lane.  + slot (kMaxA = any lane bit)
+  // per lane.  + slot or kMaxA + lane bit
   CODE_MUXL = CODE_DENSE2 + kMaxA * kMaxA,
   // MUX on a register slot whose pattern lies on register slots only: the
   // matrix is chosen per register pair, uniformly over the wave.  + slot
-  CODE_MUXR = CODE_MUXL + kMaxA + 1,
-  // MUX on a lane qubit whose pattern is one register slot: + that slot
+  CODE_MUXR = CODE_MUXL + kLaneCodes,
+  // MUX on a lane qubit whose pattern is one register slot JC:
+  // + JC * kLaneBits + lane bit
   CODE_MUXRL = CODE_MUXR + kMaxA,
   // ops on a wave bit (the slot's wave bit is s0 - kWaveSlot)
-  CODE_U1W = CODE_MUXRL + kMaxA,
+  CODE_U1W = CODE_MUXRL + kMaxA * kLaneBits,
   CODE_C1W = CODE_U1W + 1,
   CODE_X1W = CODE_C1W + 1,
   CODE_MUXW = CODE_X1W + 1  // pattern on no register slot
@@ -104,7 +108,7 @@ enum : uint32_t {
 // `mux_form`: one of MUX_LANES, MUX_REGS, MUX_ONE_REG for OP_MUX.
 __host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t s1,
                                             uint32_t mux_form = MUX_LANES) {
-  const uint32_t j0 = s0 >= kLaneSlot ? (uint32_t)kMaxA : s0;
+  const uint32_t j0 = s0 >= kLaneSlot ? (uint32_t)kMaxA + (s0 - kLaneSlot) : s0;
   if (s0 >= kWaveSlot) {
     switch (type) {
       case OP_U1: return CODE_U1W;
@@ -116,7 +120,7 @@ __host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t
   }
   switch (type) {
     case OP_MUX:
-      if (mux_form == MUX_ONE_REG) return CODE_MUXRL + s1;
+      if (mux_form == MUX_ONE_REG) return CODE_MUXRL + s1 * kLaneBits + (s0 - kLaneSlot);
       return (mux_form == MUX_REGS ? CODE_MUXR : CODE_MUXL) + j0;
     case OP_U1: return CODE_U1 + j0;
     case OP_C1: return CODE_C1 + j0;
@@ -230,16 +234,89 @@ __device__ __forceinline__ bool pred_on(const Pred<I> &p, const Regs<A, I> &g, i
   return p.lane_ok && ((goff<A>(g, r) & p.cmr) == p.cvr);
 }
 
-// Partner lane's value: `addr` = 4 * (lane ^ (1 << k)).
-__device__ __forceinline__ double xchg(double v, int addr) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffll));
-  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+// Partner lane's value (lane ^ 2^K) by VALU cross-lane instructions: DPP
+// for lane bits 0..3 (bit 2 takes two moves: row_half_mirror is lane ^ 7 in
+// groups of 8, quad_perm [3,2,1,0] is lane ^ 3), the gfx950 half swaps for
+// bits 4 and 5 (swapping a value with itself leaves the lower half's values
+// in the first result and the upper half's in the second).  ds_bpermute,
+// which these replace, goes through the CU's LDS crossbar: measured 24
+// cycles per wave instruction per SIMD at full occupancy, against 4.5 for
+// these.
+template <int K>
+__device__ __forceinline__ int lane_xor(int v, bool hi) {
+  static_assert(K >= 0 && K < kLaneBits, "lane bit");
+  if constexpr (K == 0) {
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (K == 1) {
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (K == 2) {
+    const int m = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return __builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, false);         // quad_perm [3,2,1,0]
+  } else if constexpr (K == 3) {
+    return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  } else if constexpr (K == 4) {
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return hi ? p[0] : p[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return hi ? p[0] : p[1];
+  }
 }
 
-__device__ __forceinline__ float xchg(float v, int addr) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+template <int K>
+__device__ __forceinline__ double lane_xor(double v, bool hi) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_xor<K>((int)(b & 0xffffffffll), hi);
+  const int h = lane_xor<K>((int)(b >> 32), hi);
+  return __longlong_as_double(((long long)h << 32) | (unsigned int)lo);
+}
+
+template <int K>
+__device__ __forceinline__ float lane_xor(float v, bool hi) {
+  return __int_as_float(lane_xor<K>(__float_as_int(v), hi));
+}
+
+// The two amplitudes of a lane's pair on lane bit K.  K < 4: the lane's own
+// amplitude (u) and its partner's (w), one DPP move.  K = 4, 5
+// (kBitOrder<K>): the amplitude with the bit clear (u) and the one with it
+// set (w), which the half swap of a value with itself hands every lane
+// directly; ops then weight them with row `hi` of their matrix, so no
+// per-register select is needed.
+template <int K>
+constexpr bool kBitOrder = K >= 4;
+
+template <int K>
+__device__ __forceinline__ void lane_pair(int v, bool hi, int &u, int &w) {
+  if constexpr (K < 4) {
+    u = v;
+    w = lane_xor<K>(v, hi);
+  } else if constexpr (K == 4) {
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    u = p[0];
+    w = p[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    u = p[0];
+    w = p[1];
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void lane_pair(double v, bool hi, double &u, double &w) {
+  const long long b = __double_as_longlong(v);
+  int ul, uh, wl, wh;
+  lane_pair<K>((int)(b & 0xffffffffll), hi, ul, wl);
+  lane_pair<K>((int)(b >> 32), hi, uh, wh);
+  u = __longlong_as_double(((long long)uh << 32) | (unsigned int)ul);
+  w = __longlong_as_double(((long long)wh << 32) | (unsigned int)wl);
+}
+
+template <int K>
+__device__ __forceinline__ void lane_pair(float v, bool hi, float &u, float &w) {
+  int ui, wi;
+  lane_pair<K>(__float_as_int(v), hi, ui, wi);
+  u = __int_as_float(ui);
+  w = __int_as_float(wi);
 }
 
 template <typename R, int NA>
@@ -311,25 +388,28 @@ __device__ __forceinline__ void op_1q_reg(Amp<R, 1 << A> &a, const QS_CONST OpRe
 // 2x2 on a lane qubit: a lane with the bit clear holds x and gets the
 // partner's y: out = m00 x + m01 y; a lane with the bit set holds y and
 // computes out = m10 x + m11 y from the partner's x.
-template <bool kCtl, int A, typename R, typename I>
+template <bool kCtl, int K, int A, typename R, typename I>
 __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                            const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
-  const int k = (int)op.s0 - (int)kLaneSlot;
-  const bool hi = (c.lane >> k) & 1;
-  const int addr = (c.lane ^ (1 << k)) << 2;
+  const bool hi = (c.lane >> K) & 1;
   R mv[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) mv[k] = sld(m + k);
-  const R sr = hi ? mv[6] : mv[0], si = hi ? mv[7] : mv[1];
-  const R pr = hi ? mv[4] : mv[2], pi = hi ? mv[5] : mv[3];
+  for (int q = 0; q < 8; ++q) mv[q] = sld(m + q);
+  // coefficients of the pair values u, w: row hi of the matrix, in the
+  // order lane_pair delivers them
+  constexpr bool ord = kBitOrder<K>;
+  const R sr = hi ? (ord ? mv[4] : mv[6]) : mv[0], si = hi ? (ord ? mv[5] : mv[7]) : mv[1];
+  const R pr = hi ? (ord ? mv[6] : mv[4]) : mv[2], pi = hi ? (ord ? mv[7] : mv[5]) : mv[3];
   Pred<I> p{};
   if constexpr (kCtl) p = make_pred<A>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const R xr = a.re[r], xi = a.im[r];
-    const R yr = xchg(xr, addr), yi = xchg(xi, addr);
+    R vr, vi, yr, yi;
+    lane_pair<K>(xr, hi, vr, yr);
+    lane_pair<K>(xi, hi, vi, yi);
     R ur = 0, ui = 0;
-    cfma(ur, ui, sr, si, xr, xi);
+    cfma(ur, ui, sr, si, vr, vi);
     cfma(ur, ui, pr, pi, yr, yi);
     if constexpr (kCtl) {
       const bool on = pred_on<A>(p, g, r);
@@ -339,9 +419,6 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
       a.re[r] = ur;
       a.im[r] = ui;
     }
-    // exchanges of kXchgBatch registers in flight at a time: enough to hide
-    // the LDS latency, few enough to bound the extra registers
-    if ((r % kXchgBatch) == kXchgBatch - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -408,34 +485,35 @@ __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpR
 
 // Multiplexed 2x2 on a lane qubit; the pattern has no register-slot bit, so
 // each lane picks its matrix row once for the whole op.
-template <int A, typename R, typename I>
+template <int K, int A, typename R, typename I>
 __device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                             const QS_CONST R *m, const Ctx<I> &c,
                                             const Regs<A, I> &g) {
   const Pred<I> p = make_pred<A>(op, c, g);
-  const int k = (int)op.s0 - (int)kLaneSlot;
-  const bool hi = (c.lane >> k) & 1;
-  const int addr = (c.lane ^ (1 << k)) << 2;
+  const bool hi = (c.lane >> K) & 1;
   // this lane's output row: the coefficient of its own amplitude (s) and of
   // the partner's (p); selects keep the matrix on scalar loads
   const bool on = p.lane_ok;
   R mv[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) mv[k] = sld(m + k);
-  const R sr = hi ? (on ? mv[14] : mv[6]) : (on ? mv[8] : mv[0]);
-  const R si = hi ? (on ? mv[15] : mv[7]) : (on ? mv[9] : mv[1]);
-  const R pr = hi ? (on ? mv[12] : mv[4]) : (on ? mv[10] : mv[2]);
-  const R pi = hi ? (on ? mv[13] : mv[5]) : (on ? mv[11] : mv[3]);
+  for (int q = 0; q < 16; ++q) mv[q] = sld(m + q);
+  // coefficients of the pair values u, w (see lane_pair)
+  constexpr bool ord = kBitOrder<K>;
+  const int su = ord ? 4 : 6, sw = ord ? 6 : 4;
+  const R sr = hi ? (on ? mv[8 + su] : mv[su]) : (on ? mv[8] : mv[0]);
+  const R si = hi ? (on ? mv[9 + su] : mv[su + 1]) : (on ? mv[9] : mv[1]);
+  const R pr = hi ? (on ? mv[8 + sw] : mv[sw]) : (on ? mv[10] : mv[2]);
+  const R pi = hi ? (on ? mv[9 + sw] : mv[sw + 1]) : (on ? mv[11] : mv[3]);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
-    const R xr = a.re[r], xi = a.im[r];
-    const R yr = xchg(xr, addr), yi = xchg(xi, addr);
+    R xr, xi, yr, yi;
+    lane_pair<K>(a.re[r], hi, xr, yr);
+    lane_pair<K>(a.im[r], hi, xi, yi);
     R ur = 0, ui = 0;
     cfma(ur, ui, sr, si, xr, xi);
     cfma(ur, ui, pr, pi, yr, yi);
     a.re[r] = ur;
     a.im[r] = ui;
-    if ((r % kXchgBatch) == kXchgBatch - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -443,13 +521,11 @@ __device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST Op
 // the register pairs with slot bit JC set use one matrix, the others the
 // other, so each lane picks its two row pairs once and every register's
 // choice is fixed at compile time.
-template <int JC, int A, typename R, typename I>
+template <int JC, int K, int A, typename R, typename I>
 __device__ __forceinline__ void op_mux_lane_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                                 const QS_CONST R *m, const Ctx<I> &c,
                                                 const Regs<A, I> &g) {
-  const int k = (int)op.s0 - (int)kLaneSlot;
-  const bool hi = (c.lane >> k) & 1;
-  const int addr = (c.lane ^ (1 << k)) << 2;
+  const bool hi = (c.lane >> K) & 1;
   // the pattern holds where slot JC's bit equals its value in cval
   const bool v1 = ((I)op.cval & g.gb[JC]) != 0;
   const QS_CONST R *mb1 = v1 ? m + 8 : m;  // matrix where the slot bit is 1
@@ -460,26 +536,29 @@ __device__ __forceinline__ void op_mux_lane_reg(Amp<R, 1 << A> &a, const QS_CONS
     w0[q] = sld(mb0 + q);
     w1[q] = sld(mb1 + q);
   }
+  // coefficients of the pair values u, w (see lane_pair)
+  constexpr bool ord = kBitOrder<K>;
+  constexpr int su = ord ? 4 : 6, sw = ord ? 6 : 4;
   R sr[2], si[2], pr[2], pi[2];
-  sr[0] = hi ? w0[6] : w0[0];
-  si[0] = hi ? w0[7] : w0[1];
-  pr[0] = hi ? w0[4] : w0[2];
-  pi[0] = hi ? w0[5] : w0[3];
-  sr[1] = hi ? w1[6] : w1[0];
-  si[1] = hi ? w1[7] : w1[1];
-  pr[1] = hi ? w1[4] : w1[2];
-  pi[1] = hi ? w1[5] : w1[3];
+  sr[0] = hi ? w0[su] : w0[0];
+  si[0] = hi ? w0[su + 1] : w0[1];
+  pr[0] = hi ? w0[sw] : w0[2];
+  pi[0] = hi ? w0[sw + 1] : w0[3];
+  sr[1] = hi ? w1[su] : w1[0];
+  si[1] = hi ? w1[su + 1] : w1[1];
+  pr[1] = hi ? w1[sw] : w1[2];
+  pi[1] = hi ? w1[sw + 1] : w1[3];
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
+    R xr, xi, yr, yi;
+    lane_pair<K>(a.re[r], hi, xr, yr);
+    lane_pair<K>(a.im[r], hi, xi, yi);
     const int b = (r >> JC) & 1;
-    const R xr = a.re[r], xi = a.im[r];
-    const R yr = xchg(xr, addr), yi = xchg(xi, addr);
     R ur = 0, ui = 0;
     cfma(ur, ui, sr[b], si[b], xr, xi);
     cfma(ur, ui, pr[b], pi[b], yr, yi);
     a.re[r] = ur;
     a.im[r] = ui;
-    if ((r % kXchgBatch) == kXchgBatch - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -595,20 +674,17 @@ __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRe
   }
 }
 
-template <int A, typename R, typename I>
+template <int K, int A, typename R, typename I>
 __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                            const Ctx<I> &c, const Regs<A, I> &g) {
-  const int k = (int)op.s0 - (int)kLaneSlot;
-  const int addr = (c.lane ^ (1 << k)) << 2;
+  const bool hi = (c.lane >> K) & 1;
   const Pred<I> p = make_pred<A>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
-    const R xr = a.re[r], xi = a.im[r];
-    const R yr = xchg(xr, addr), yi = xchg(xi, addr);
+    const R yr = lane_xor<K>(a.re[r], hi), yi = lane_xor<K>(a.im[r], hi);
     const bool on = pred_on<A>(p, g, r);
-    a.re[r] = on ? yr : xr;
-    a.im[r] = on ? yi : xi;
-    if ((r % kXchgBatch) == kXchgBatch - 1) __builtin_amdgcn_sched_barrier(0);
+    a.re[r] = on ? yr : a.re[r];
+    a.im[r] = on ? yi : a.im[r];
   }
 }
 
@@ -708,6 +784,19 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       op_dense2<(J0), (J1), A>(a, op, m, c, g);                              \
     break;
 #define QS_REG2_ROW(J0) QS_REG2(J0, 0) QS_REG2(J0, 1) QS_REG2(J0, 2) QS_REG2(J0, 3) QS_REG2(J0, 4)
+  // one case per lane bit LK = 0..5 at codes BASE + LK
+#define QS_LANE1(BASE, K, CALL) \
+  case (BASE) + (K): {          \
+    constexpr int LK = (K);     \
+    CALL;                       \
+    break;                      \
+  }
+#define QS_LANE6(BASE, CALL)                                                       \
+  QS_LANE1(BASE, 0, CALL) QS_LANE1(BASE, 1, CALL) QS_LANE1(BASE, 2, CALL)          \
+  QS_LANE1(BASE, 3, CALL) QS_LANE1(BASE, 4, CALL) QS_LANE1(BASE, 5, CALL)
+#define QS_MUXRL6(JC)                                                              \
+  QS_LANE6(CODE_MUXRL + (JC) * kLaneBits,                                          \
+           if constexpr ((JC) < A) (op_mux_lane_reg<(JC), LK, A>(a, op, m, c, g)))
   for (int o = 0; o < nops; ++o) {
     const QS_CONST OpRec &op = ops[o];
     const QS_CONST R *m = pool + op.moff;
@@ -717,19 +806,19 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       QS_REG1(CODE_U1, 2, (op_1q_reg<2, false, A>(a, op, m, c, g)))
       QS_REG1(CODE_U1, 3, (op_1q_reg<3, false, A>(a, op, m, c, g)))
       QS_REG1(CODE_U1, 4, (op_1q_reg<4, false, A>(a, op, m, c, g)))
-      case CODE_U1 + kMaxA: op_1q_lane<false, A>(a, op, m, c, g); break;
+      QS_LANE6(CODE_U1 + kMaxA, (op_1q_lane<false, LK, A>(a, op, m, c, g)))
       QS_REG1(CODE_C1, 0, (op_1q_reg<0, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_C1, 1, (op_1q_reg<1, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_C1, 2, (op_1q_reg<2, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_C1, 3, (op_1q_reg<3, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_C1, 4, (op_1q_reg<4, true, A>(a, op, m, c, g)))
-      case CODE_C1 + kMaxA: op_1q_lane<true, A>(a, op, m, c, g); break;
+      QS_LANE6(CODE_C1 + kMaxA, (op_1q_lane<true, LK, A>(a, op, m, c, g)))
       QS_REG1(CODE_X1, 0, (op_x1_reg<0, A, R, I>(a, op, c, g)))
       QS_REG1(CODE_X1, 1, (op_x1_reg<1, A, R, I>(a, op, c, g)))
       QS_REG1(CODE_X1, 2, (op_x1_reg<2, A, R, I>(a, op, c, g)))
       QS_REG1(CODE_X1, 3, (op_x1_reg<3, A, R, I>(a, op, c, g)))
       QS_REG1(CODE_X1, 4, (op_x1_reg<4, A, R, I>(a, op, c, g)))
-      case CODE_X1 + kMaxA: op_x1_lane<A, R, I>(a, op, c, g); break;
+      QS_LANE6(CODE_X1 + kMaxA, (op_x1_lane<LK, A, R, I>(a, op, c, g)))
       case CODE_SPARSE: op_sparse<A>(a, op, m, c, g); break;
       case CODE_DIAG: op_diag<A>(a, op, m, c, g); break;
       QS_REG2_ROW(0)
@@ -742,17 +831,17 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       QS_REG1(CODE_MUXL, 2, (op_mux_reg<2, false, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXL, 3, (op_mux_reg<3, false, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXL, 4, (op_mux_reg<4, false, A>(a, op, m, c, g)))
-      case CODE_MUXL + kMaxA: op_mux_lane<A>(a, op, m, c, g); break;
+      QS_LANE6(CODE_MUXL + kMaxA, (op_mux_lane<LK, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXR, 0, (op_mux_reg<0, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXR, 1, (op_mux_reg<1, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXR, 2, (op_mux_reg<2, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXR, 3, (op_mux_reg<3, true, A>(a, op, m, c, g)))
       QS_REG1(CODE_MUXR, 4, (op_mux_reg<4, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXRL, 0, (op_mux_lane_reg<0, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXRL, 1, (op_mux_lane_reg<1, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXRL, 2, (op_mux_lane_reg<2, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXRL, 3, (op_mux_lane_reg<3, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXRL, 4, (op_mux_lane_reg<4, A>(a, op, m, c, g)))
+      QS_MUXRL6(0)
+      QS_MUXRL6(1)
+      QS_MUXRL6(2)
+      QS_MUXRL6(3)
+      QS_MUXRL6(4)
       case CODE_U1W:
         if constexpr (W > 0) op_wave<0, A>(a, op, m, c, g);
         break;
@@ -768,6 +857,9 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       default: break;
     }
   }
+#undef QS_MUXRL6
+#undef QS_LANE6
+#undef QS_LANE1
 #undef QS_REG2_ROW
 #undef QS_REG2
 #undef QS_REG1

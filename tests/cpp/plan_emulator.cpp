@@ -129,22 +129,27 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
   };
   for (const OpRec &o : ops) {
     const uint32_t code = o.code;
-    if (g_hist && code < 64) ++g_hist[code];
+    if (g_hist && code < 128) ++g_hist[code];
     int kind = 0, j = 0;
-    if (code >= CODE_U1 && code <= CODE_U1 + kMaxA) kind = OP_U1, j = code - CODE_U1;
-    else if (code >= CODE_C1 && code <= CODE_C1 + kMaxA) kind = OP_C1, j = code - CODE_C1;
-    else if (code >= CODE_X1 && code <= CODE_X1 + kMaxA) kind = OP_X1, j = code - CODE_X1;
+    int lane_code = -1;  // lane bit a lane-target code names
+    if (code >= CODE_U1 && code < CODE_U1 + kLaneCodes) kind = OP_U1, j = code - CODE_U1;
+    else if (code >= CODE_C1 && code < CODE_C1 + kLaneCodes) kind = OP_C1, j = code - CODE_C1;
+    else if (code >= CODE_X1 && code < CODE_X1 + kLaneCodes) kind = OP_X1, j = code - CODE_X1;
     else if (code == CODE_SPARSE) kind = OP_SPARSE;
     else if (code == CODE_DIAG) kind = OP_DIAG;
     else if (code >= CODE_DENSE2 && code < CODE_MUXL) kind = OP_DENSE2, j = code - CODE_DENSE2;
-    else if (code >= CODE_MUXL && code <= CODE_MUXL + kMaxA) kind = 100, j = code - CODE_MUXL;
+    else if (code >= CODE_MUXL && code < CODE_MUXL + kLaneCodes) kind = 100, j = code - CODE_MUXL;
     else if (code >= CODE_MUXR && code < CODE_MUXR + kMaxA) kind = 101, j = code - CODE_MUXR;
-    else if (code >= CODE_MUXRL && code < CODE_MUXRL + kMaxA) kind = 102, j = code - CODE_MUXRL;
+    else if (code >= CODE_MUXRL && code < CODE_MUXRL + kMaxA * kLaneBits)
+      kind = 102, j = (code - CODE_MUXRL) / kLaneBits, lane_code = (code - CODE_MUXRL) % kLaneBits;
     else if (code == CODE_U1W) kind = OP_U1, j = -1;
     else if (code == CODE_C1W) kind = OP_C1, j = -1;
     else if (code == CODE_X1W) kind = OP_X1, j = -1;
     else if (code == CODE_MUXW) kind = 100, j = -1;
     else return fail("unknown op code " + std::to_string(code));
+    if (kind != 102 && kind != OP_DENSE2 && j >= kMaxA) lane_code = j - kMaxA, j = kMaxA;
+    if (lane_code >= 0 && lane_code != (int)o.s0 - (int)kLaneSlot)
+      return fail("lane bit of the code and of the op slot disagree");
     const uint64_t cm = o.cmask;
     const uint64_t cv = o.cval & cm;
     auto slot_qubit = [&](int jj, uint32_t s0) -> int {
@@ -264,7 +269,7 @@ extern "C" {
 // run by the host per-gate path, sweep op records, merged gates.  Returns 0,
 // or 1 with a message in err.
 // psi == NULL: plan only (descriptors are decoded and checked, no state).
-// hist (64 entries, may be NULL) counts the sweep op records per op code.
+// hist (128 entries, may be NULL) counts the sweep op records per op code.
 // cmasks/cvals (may be NULL): controls of each gate beyond its matrix, as
 // qs_apply_controlled passes them to the classifier.
 int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const uint64_t *targets,
