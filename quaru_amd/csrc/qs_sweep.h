@@ -72,7 +72,7 @@ int sweep_low_bits(bool c64);
 int sweep_page_qubit(bool c64);
 
 // Whether sweeps use workgroup tiles whose 4 waves add two tile qubits
-// (QS_SWEEP_WAVES_C128 / QS_SWEEP_WAVES_C64, defaults on / off).
+// (QS_SWEEP_WAVES_C128 / QS_SWEEP_WAVES_C64, both on by default).
 bool sweep_wave_bits(bool c64);
 
 // How many of a workgroup tile's wave bits take dense gates

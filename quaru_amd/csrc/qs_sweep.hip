@@ -621,10 +621,10 @@ int sweep_low_bits(bool c64) {
 // Workgroup tiles with wave bits (QS_SWEEP_WAVES_C128 / _C64, default on).
 bool sweep_wave_bits(bool c64) {
   static const bool on128 = env_int("QS_SWEEP_WAVES_C128", 1) != 0;
-  // complex64 is bound by arithmetic: measured on the 30-qubit circuit, 61
-  // workgroup-tile sweeps of 4.58 ms give the rate of 79 wave-tile sweeps of
-  // 3.51 ms, so complex64 keeps tiles per wave
-  static const bool on64 = env_int("QS_SWEEP_WAVES_C64", 0) != 0;
+  // complex64 is bound by arithmetic more than complex128: measured on the
+  // 30-qubit circuit, 68 workgroup-tile sweeps of 3.94 ms (4880 gates/s)
+  // against 79 wave-tile sweeps of 3.45 ms (4780-4790 gates/s)
+  static const bool on64 = env_int("QS_SWEEP_WAVES_C64", 1) != 0;
   return c64 ? on64 : on128;
 }
 

@@ -152,9 +152,9 @@ def test_plan_sizes_of_the_benchmark_workloads(emu):
     """Plan-only runs of the 30-qubit random circuit and the 32-qubit Grover
     workload (no state): the sweep counts the planner reaches, as guards
     against planning regressions (holding gates back in front of a
-    register-wide phase once took Grover from 100 sweeps to 268).  complex128
-    registers of 12 or more qubits use workgroup tiles (one wave bit open to
-    dense gates), complex64 ones tiles per wave."""
+    register-wide phase once took Grover from 100 sweeps to 268).  Registers
+    of 12 (complex128) or 13 (complex64) or more qubits use workgroup tiles
+    (one wave bit open to dense gates)."""
     w = circuits.random_circuit(30, 20, seed=2025, triples=8, measure=False)
     _, st = emu(30, "c128", None, w.ops)
     assert st["sweeps"] <= 74 and st["merged"] >= 550, st
