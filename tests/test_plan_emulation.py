@@ -71,12 +71,16 @@ def emu(tmp_path_factory):
         cms = np.ascontiguousarray(cms, dtype=np.uint64)
         cvs = np.ascontiguousarray(cvs, dtype=np.uint64)
         stats = np.zeros(4, dtype=np.uint64)
+        hist = np.zeros(128, dtype=np.uint64)
         err = ctypes.create_string_buffer(512)
         rc = lib.emu_run(n, 1 if dtype == "c64" else 0, None if st is None else st.ctypes.data,
                          len(opl), mats.ctypes.data, tg.ctypes.data, ks.ctypes.data,
-                         cms.ctypes.data, cvs.ctypes.data, stats.ctypes.data, None, err, 512)
+                         cms.ctypes.data, cvs.ctypes.data, stats.ctypes.data, hist.ctypes.data,
+                         err, 512)
         assert rc == 0, err.value.decode()
-        return st, dict(zip(["sweeps", "singles", "sweep_ops", "merged"], map(int, stats)))
+        out = dict(zip(["sweeps", "singles", "sweep_ops", "merged"], map(int, stats)))
+        out["hist"] = hist
+        return st, out
     return run
 
 
@@ -99,6 +103,36 @@ def test_merge_stress_plans(emu, n, dtype):
     ref = oracle_run(psi.copy(), opl)
     assert st["sweeps"] > 0 and st["merged"] > 40
     assert np.max(np.abs(got - ref)) < TOL[dtype]
+
+
+# Dispatch codes of the lane-target ops, as qs_sweep_kernel.h numbers them:
+# per one-qubit form 5 register slots then 6 lane bits; the one-register-
+# pattern multiplexed form at CODE_MUXRL + slot * 6 + lane bit.
+_LANE_CODES = 5 + 6
+CODE_U1 = 1
+CODE_C1 = CODE_U1 + _LANE_CODES
+CODE_X1 = CODE_C1 + _LANE_CODES
+CODE_MUXL = CODE_X1 + _LANE_CODES + 2 + 25
+CODE_MUXRL = CODE_MUXL + _LANE_CODES + 5
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_gpu_workloads_reach_every_lane_bit(emu, dtype):
+    """The sweep kernel compiles each lane-target op once per lane bit (DPP
+    moves for bits 0-3, permlane swaps for 4 and 5).  The merge-stress
+    workloads of the GPU parity test (tests/test_gpu_register.py) must reach
+    every lane bit of every one-qubit form, and lane bits 4 and 5 of the
+    one-register-pattern form, so the GPU tests run each of those paths."""
+    hist = np.zeros(128, dtype=np.uint64)
+    for n in (11, 14):
+        rng = np.random.default_rng(77 + n)
+        _, st = emu(n, dtype, None, merge_stress_ops(rng, n, 400))
+        hist += st["hist"]
+    for name, base in (("U1", CODE_U1), ("C1", CODE_C1), ("X1", CODE_X1), ("MUXL", CODE_MUXL)):
+        missing = [k for k in range(6) if hist[base + 5 + k] == 0]
+        assert not missing, f"{name}: lane bits {missing} never planned"
+    for k in (4, 5):
+        assert sum(int(hist[CODE_MUXRL + j * 6 + k]) for j in range(5)) > 0, k
 
 
 @pytest.mark.parametrize("n,layers,triples", [(12, 4, 0), (14, 3, 4), (16, 2, 8), (17, 2, 8)])
