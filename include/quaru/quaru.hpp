@@ -670,14 +670,22 @@ class Register {
   // Register::apply_circuit kept on the device: measurements decided on the
   // GPU (same uniforms, same rule as measure) and every conditional
   // operation applied as an operation controlled on its condition's measured
-  // qubits.  Exact, because a circuit's conditional operations never act on
-  // measured qubits (src/operation.rs:302-321): after the collapse, "target
-  // measured v" holds exactly where qubit `target` is v.  One
+  // qubits.  After the collapse, "target measured v" holds exactly where
+  // qubit `target` is v, as long as no conditional operation acts on a
+  // measured qubit.  add_conditional_operation only refuses targets measured
+  // before it (src/operation.rs:302-321); a later add_measurement may name a
+  // conditional operation's target, and then a condition on that qubit must
+  // read the recorded outcome: such circuits (and those with more
+  // measurements than device slots) are decided on the host.  One
   // synchronisation at the end reads the outcomes.
   std::vector<std::pair<size_t, bool>> apply_circuit(QuantumCircuit &circuit) {
     for (const Operation &op : circuit.get_operations()) apply(op);
     const std::vector<size_t> targets = circuit.get_measurement_targets();
-    if (targets.size() > QS_MEASURE_SLOTS) return apply_circuit_host(circuit);
+    bool host = targets.size() > QS_MEASURE_SLOTS;
+    for (const auto &cop : circuit.get_conditional_operations())
+      for (size_t t : cop.first.targets())
+        for (size_t m : targets) host = host || t == m;
+    if (host) return apply_circuit_host(circuit);
     for (size_t j = 0; j < targets.size(); ++j)
       detail::check(qs_measure_deferred(h_, targets[j], rng_.uniform(), j));
     for (const auto &cop : circuit.get_conditional_operations()) {
@@ -717,7 +725,7 @@ class Register {
   }
 
   // The same with the outcomes decided on the host, one measure at a time
-  // (more measurements than device slots).
+  // (src/register.rs:345-366 as written).
   std::vector<std::pair<size_t, bool>> apply_circuit_host(QuantumCircuit &circuit) {
     std::vector<std::pair<size_t, bool>> measures;
     if (!circuit.get_measurement_targets().empty()) measures = measure_circuit(circuit);
@@ -824,5 +832,140 @@ class Register {
   qs_register *h_ = nullptr;
   detail::SplitMix64 rng_;
 };
+
+
+// ------------------------------------------------------------- adjoint ----
+// Gradients of E(theta) = <psi(theta)|H|psi(theta)> by the adjoint method,
+// behind a Function-style forward/backward pair.  No reference counterpart:
+// quaru has no parameterised gates; the rotations follow its OpenQASM U gate
+// (src/openqasm.rs:145-163): RY(t) = u(t, 0, 0), RZ(t) = u(0, 0, t), and a
+// circuit's fixed gates are the reference's Operations (src/operation.rs).
+namespace adjoint {
+
+enum class Axis { X = QS_AXIS_X, Y = QS_AXIS_Y, Z = QS_AXIS_Z };
+
+// Fixed gates (k <= 2) and rotations exp(-i theta_p sigma / 2), with layer
+// marks (the checkpoint granularity).
+class ParamCircuit {
+ public:
+  explicit ParamCircuit(size_t n) : n_(n) { detail::check(qs_pcircuit_new(n, &h_)); }
+  ParamCircuit(const ParamCircuit &) = delete;
+  ParamCircuit &operator=(const ParamCircuit &) = delete;
+  ParamCircuit(ParamCircuit &&o) noexcept : h_(o.h_), n_(o.n_) { o.h_ = nullptr; }
+  ~ParamCircuit() { qs_pcircuit_free(h_); }
+
+  // A fixed gate: the Operation's dense matrix on its targets.
+  ParamCircuit &gate(const Operation &op) {
+    const Matrix m = op.matrix();
+    const std::vector<size_t> t = op.targets();
+    detail::check(qs_pcircuit_gate(h_, reinterpret_cast<const double *>(m.data.data()), t.data(),
+                                   t.size()));
+    return *this;
+  }
+  // theta_param about `axis` on `target`.
+  ParamCircuit &rotation(Axis axis, size_t target, size_t param) {
+    detail::check(qs_pcircuit_rotation(h_, static_cast<int>(axis), target, param));
+    return *this;
+  }
+  ParamCircuit &ry(size_t target, size_t param) { return rotation(Axis::Y, target, param); }
+  ParamCircuit &rz(size_t target, size_t param) { return rotation(Axis::Z, target, param); }
+  ParamCircuit &rx(size_t target, size_t param) { return rotation(Axis::X, target, param); }
+  ParamCircuit &layer() {
+    detail::check(qs_pcircuit_layer(h_));
+    return *this;
+  }
+  size_t nparams() const { return qs_pcircuit_nparams(h_); }
+  size_t size() const { return n_; }
+  const qs_pcircuit *handle() const { return h_; }
+
+ private:
+  qs_pcircuit *h_ = nullptr;
+  size_t n_ = 0;
+};
+
+// H = sum_t coeffs[t] Z^{masks[t]}: term t contributes
+// coeffs[t] * (-1)^popcount(x & masks[t]) on basis state x.
+struct Observable {
+  std::vector<uint64_t> masks;
+  std::vector<double> coeffs;
+};
+
+// forward(thetas) -> energies; backward(grad_energies) -> grad_thetas.
+// thetas and gradients are row-major batch x nparams.  A forward keeps the
+// batch's final states (and a checkpoint every `checkpoint_every` layers) on
+// the device until the next backward or forward.
+class Expectation {
+ public:
+  Expectation(const ParamCircuit &circuit, Observable h, Dtype dtype = Dtype::C128,
+              size_t checkpoint_every = 0)
+      : c_(&circuit), h_(std::move(h)), dtype_(dtype), ckpt_(checkpoint_every) {
+    if (h_.masks.size() != h_.coeffs.size())
+      throw std::invalid_argument("Observable: masks and coeffs differ in length");
+  }
+  Expectation(const Expectation &) = delete;
+  Expectation &operator=(const Expectation &) = delete;
+  ~Expectation() { release(); }
+
+  // Energies of every parameter set; nothing stays on the device.
+  std::vector<double> energies(const std::vector<double> &thetas) const {
+    const size_t b = batch_of(thetas);
+    std::vector<double> e(b);
+    detail::check(qs_adjoint_forward(c_->handle(), thetas.data(), b, h_.masks.data(),
+                                     h_.coeffs.data(), h_.masks.size(), (int)dtype_, 0, nullptr,
+                                     e.data()));
+    return e;
+  }
+
+  std::vector<double> forward(const std::vector<double> &thetas) {
+    release();
+    batch_ = batch_of(thetas);
+    std::vector<double> e(batch_);
+    detail::check(qs_adjoint_forward(c_->handle(), thetas.data(), batch_, h_.masks.data(),
+                                     h_.coeffs.data(), h_.masks.size(), (int)dtype_, ckpt_, &ctx_,
+                                     e.data()));
+    return e;
+  }
+
+  // dE_b / dtheta_{b,p} (batch x nparams), one reverse sweep.
+  std::vector<double> backward() {
+    if (!ctx_) throw Panic("backward() needs a forward() first");
+    std::vector<double> g(batch_ * c_->nparams());
+    const int rc = qs_adjoint_backward(ctx_, g.data());
+    release();
+    detail::check(rc);
+    return g;
+  }
+  // The vector-Jacobian product: grad_energies[b] * dE_b / dtheta_{b,p}.
+  std::vector<double> backward(const std::vector<double> &grad_energies) {
+    if (grad_energies.size() != batch_)
+      throw std::invalid_argument("backward: one incoming gradient per parameter set");
+    std::vector<double> g = backward();
+    const size_t p = c_->nparams();
+    for (size_t b = 0; b < grad_energies.size(); ++b)
+      for (size_t j = 0; j < p; ++j) g[b * p + j] *= grad_energies[b];
+    return g;
+  }
+
+ private:
+  size_t batch_of(const std::vector<double> &thetas) const {
+    const size_t p = c_->nparams();
+    if (p == 0 || thetas.size() % p != 0 || thetas.empty())
+      throw std::invalid_argument("thetas: need batch x nparams values");
+    return thetas.size() / p;
+  }
+  void release() {
+    if (ctx_) qs_adjoint_free(ctx_);
+    ctx_ = nullptr;
+  }
+
+  const ParamCircuit *c_;
+  Observable h_;
+  Dtype dtype_;
+  size_t ckpt_;
+  qs_adjoint *ctx_ = nullptr;
+  size_t batch_ = 0;
+};
+
+}  // namespace adjoint
 
 }  // namespace quaru

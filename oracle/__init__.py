@@ -62,8 +62,6 @@ def lib():
         L.orc_try_measure.argtypes = [P, S, S, ctypes.c_double, ctypes.POINTER(ctypes.c_int), SP]
         L.orc_measure_sequence.argtypes = [P, S, P, S, P]
         L.orc_measure_sequence.restype = S
-        L.orc_norm_sqr.argtypes = [P, S]
-        L.orc_norm_sqr.restype = ctypes.c_double
         L.orc_apply_fast_omp.argtypes = [P, P, S, P, S, P, ctypes.c_int]
         L.orc_expect_zstrings.argtypes = [P, S, P, P, S]
         L.orc_expect_zstrings.restype = ctypes.c_double
@@ -173,11 +171,6 @@ def measure_sequence(state, order, uniforms):
     assert u.size >= o.size
     idx = lib().orc_measure_sequence(_ptr(state), _nqubits(state), _ptr(o), o.size, _ptr(u))
     return int(idx), state
-
-
-def norm_sqr(state) -> float:
-    state = _c128(state)
-    return float(lib().orc_norm_sqr(_ptr(state), _nqubits(state)))
 
 
 def apply_fast_omp(state: np.ndarray, scratch: np.ndarray, matrix, targets, threads: int) -> None:

@@ -12,18 +12,34 @@
  * Compile with -ffp-contract=off so that a*b+c is never fused: Rust does not
  * contract either, so the per-operation rounding matches num-complex.
  *
- * Pinning (what checks each function, so a dropped term or swapped index
- * fails somewhere):
- *   orc_new / orc_from_int / orc_try_from_qubits: basis-index closed forms,
- *       the kron ordering test tests/register_operation_tests.rs:25-40.
- *   orc_try_apply_fast: dense brute force orc_try_apply (a different
- *       algorithm: permute + kron(eye, M) + permute back), numpy tensor
- *       contraction goldens, closed forms (GHZ, QFT, Grover), exhaustive
- *       modmul permutation gates (examples/shors.rs:434-463).
- *   orc_try_apply_all: per-qubit orc_try_apply_fast (tests/...:211-219).
- *   orc_try_measure: basis-state round trips (tests/...:53-96), the x > p0
- *       rule on hand-computed states, collapse normalisation.
- *   orc_sample_all_sequential: n calls of orc_try_measure (its definition).
+ * Pinning: every exported function and the test in tests/test_oracle.py that
+ * a dropped term or a swapped index fails.
+ *   orc_operation_valid: test_validation_precedence (2^k x 2^k shapes only).
+ *   orc_new / orc_from_int: test_new_is_basis_state, test_from_int (basis
+ *       index closed forms).
+ *   orc_try_from_qubits: test_from_qubits_kron_order (the kron ordering of
+ *       tests/register_operation_tests.rs:25-40), test_from_qubits_invalid
+ *       (the 1e-6 tolerance and the InvalidQubit(i) payload).
+ *   orc_try_apply_fast: test_apply_fast_matches_dense_and_tensordot (against
+ *       orc_try_apply, a different algorithm, and a numpy tensordot),
+ *       test_transposed_matrix_is_caught, test_golden_config1_12q and
+ *       test_golden_config3_mini (numpy goldens), test_grover_closed_form,
+ *       test_qft_add_identity, test_u_gate_modmul_exhaustive
+ *       (examples/shors.rs:434-503), test_toffoli_cnx.
+ *   orc_try_apply: test_apply_fast_matches_dense_and_tensordot,
+ *       test_validation_precedence.
+ *   orc_try_apply_all: test_apply_all_equals_each_qubit,
+ *       test_apply_all_random_unitary_and_arity (tests/...:211-219).
+ *   orc_try_measure: test_measure_basis_states_round_trip (tests/...:53-96),
+ *       test_measure_rule_and_collapse (x > prob_0, strict, and the
+ *       normalisation), test_bell_pairs_measure_equal,
+ *       test_teleportation_outcome_independent, test_superdense_coding.
+ *   orc_measure_sequence: test_golden_config1_12q (sampled index 3733 and the
+ *       collapsed state of the numpy golden).
+ *   orc_apply_fast_omp (qs_oracle_omp.c): test_omp_variant_matches_scalar
+ *       (bit-identical to orc_try_apply_fast).
+ *   orc_expect_zstrings (qs_oracle_omp.c): test_expect_zstrings (numpy sum of
+ *       |psi|^2 times the Z-string eigenvalues).
  *
  * Parity unpinned (nothing available here can fix them):
  *   - the uniform stream of measure: the reference draws x from the
@@ -392,12 +408,4 @@ size_t orc_measure_sequence(double *state, size_t n, const size_t *order, size_t
         if (r) idx |= (size_t)1 << order[j];
     }
     return idx;
-}
-
-/* Σ |psi_i|^2 in index order (the quantity every measure pass sums). */
-double orc_norm_sqr(const double *state, size_t n) {
-    size_t len = (size_t)1 << n;
-    double s = 0.0;
-    for (size_t i = 0; i < len; ++i) s += state[2 * i] * state[2 * i] + state[2 * i + 1] * state[2 * i + 1];
-    return s;
 }

@@ -30,7 +30,6 @@ int orc_try_apply_all(double *state, size_t n, const double *u, size_t arity, si
 int orc_try_measure(double *state, size_t n, size_t target, double x, int *res, size_t *ea);
 size_t orc_measure_sequence(double *state, size_t n, const size_t *order, size_t count,
                             const double *uniforms);
-double orc_norm_sqr(const double *state, size_t n);
 
 /* qs_oracle_omp.c: the same try_apply_fast algorithm split over OpenMP
  * threads (the multi-core CPU baseline), and the diagonal-observable

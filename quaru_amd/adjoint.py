@@ -85,6 +85,40 @@ class ParamCircuit:
         return pc
 
 
+class AdjointRun:
+    """The device state one forward pass leaves for its backward: the final
+    states (and checkpoints) of one batch.  ``backward()`` runs the reverse
+    sweep once and frees them; every forward gets its own run, so several
+    forwards may be pending at once."""
+
+    def __init__(self, ctx: ctypes.c_void_p, batch: int, nparams: int):
+        self._ctx = ctx
+        self.batch = batch
+        self.nparams = nparams
+
+    def close(self):
+        if self._ctx is not None and self._ctx.value:
+            _lib.load().qs_adjoint_free(self._ctx)
+        self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def backward(self) -> np.ndarray:
+        """dE_b / dtheta_{b,p} as a (batch, nparams) array."""
+        if self._ctx is None:
+            raise RuntimeError("backward() already ran for this forward pass")
+        g = np.zeros((self.batch, self.nparams), dtype=np.float64)
+        try:
+            _check(_lib.load().qs_adjoint_backward(self._ctx, _vp(g)))
+        finally:
+            self.close()
+        return g
+
+
 class Expectation:
     """Energies and adjoint gradients of a batch of parameter sets."""
 
@@ -97,26 +131,15 @@ class Expectation:
             raise ValueError("masks and coeffs differ in length")
         self.dtype = _dtype_code(dtype)
         self.checkpoint_every = int(checkpoint_every)
-        self._ctx = None
-        self._batch = 0
-
-    def _free(self):
-        if self._ctx is not None and self._ctx.value:
-            _lib.load().qs_adjoint_free(self._ctx)
-        self._ctx = None
-
-    def __del__(self):
-        try:
-            self._free()
-        except Exception:
-            pass
+        self._run: Optional[AdjointRun] = None
 
     def _thetas(self, thetas) -> np.ndarray:
         th = np.ascontiguousarray(np.asarray(thetas, dtype=np.float64))
         if th.ndim == 1:
             th = th.reshape(1, -1)
-        if th.shape[1] != self.circuit.nparams:
-            raise ValueError(f"expected {self.circuit.nparams} parameters, got {th.shape[1]}")
+        if th.ndim != 2 or th.shape[1] != self.circuit.nparams:
+            raise ValueError(f"expected (batch, {self.circuit.nparams}) parameters, "
+                             f"got shape {np.shape(thetas)}")
         return np.ascontiguousarray(th)
 
     def energies(self, thetas) -> np.ndarray:
@@ -128,8 +151,9 @@ class Expectation:
                                               self.dtype, 0, None, _vp(out)))
         return out
 
-    def forward(self, thetas) -> np.ndarray:
-        self._free()
+    def record(self, thetas):
+        """Forward pass that returns (energies, AdjointRun): the run holds
+        this pass's device state until its own backward()."""
         th = self._thetas(thetas)
         out = np.zeros(th.shape[0], dtype=np.float64)
         ctx = ctypes.c_void_p()
@@ -137,17 +161,22 @@ class Expectation:
                                               _vp(self.masks), _vp(self.coeffs), self.masks.size,
                                               self.dtype, self.checkpoint_every,
                                               ctypes.byref(ctx), _vp(out)))
-        self._ctx = ctx
-        self._batch = th.shape[0]
+        return out, AdjointRun(ctx, th.shape[0], self.circuit.nparams)
+
+    def forward(self, thetas) -> np.ndarray:
+        """Function-style forward: energies of the batch; the state for the
+        next backward() is kept (a new forward replaces it)."""
+        if self._run is not None:
+            self._run.close()
+        out, self._run = self.record(thetas)
         return out
 
     def backward(self) -> np.ndarray:
-        if self._ctx is None:
+        """Gradients of the last forward() as a (batch, nparams) array."""
+        if self._run is None:
             raise RuntimeError("backward() needs a forward() first")
-        g = np.zeros((self._batch, self.circuit.nparams), dtype=np.float64)
-        _check(_lib.load().qs_adjoint_backward(self._ctx, _vp(g)))
-        self._free()
-        return g
+        run, self._run = self._run, None
+        return run.backward()
 
 
 def _torch_function():
@@ -155,20 +184,26 @@ def _torch_function():
 
     class ExpectationFunction(torch.autograd.Function):
         """``E = ExpectationFunction.apply(thetas, expectation)``: thetas is a
-        (batch, nparams) float64 tensor; E has shape (batch,)."""
+        (batch, nparams) or (nparams,) float64 tensor; E has shape (batch,)
+        (one entry for a 1-D thetas).  Each application keeps its own device
+        state, so several applications may share one Expectation before a
+        single backward."""
 
         @staticmethod
         def forward(ctx, thetas, expectation: Expectation):
             th = thetas.detach().to("cpu", torch.float64).numpy()
-            e = expectation.forward(th)
-            ctx.expectation = expectation
+            e, run = expectation.record(th)
+            ctx.run = run
+            ctx.theta_shape = thetas.shape
             ctx.src_device = thetas.device
             return torch.from_numpy(e).to(thetas.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            g = torch.from_numpy(ctx.expectation.backward()).to(ctx.src_device)
-            return grad_out.to(g.dtype).unsqueeze(1) * g, None
+            g = torch.from_numpy(ctx.run.backward()).to(ctx.src_device)
+            ctx.run = None
+            grad = grad_out.to(g.dtype).unsqueeze(1) * g
+            return grad.reshape(ctx.theta_shape), None
 
     return ExpectationFunction
 

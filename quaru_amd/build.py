@@ -12,11 +12,23 @@ import shutil
 import subprocess
 import sys
 import tempfile
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libquaru_hip.so")
 SOURCES = ["qs_kernels.hip", "qs_sweep.hip", "qs_register.hip", "qs_adjoint.hip"]
+# qs_sweep_launch.hip holds the k_sweep instantiations of one (precision,
+# register bits) pair; it is compiled once per pair so that they build in
+# parallel.
+SWEEP_VARIANTS = [(r, a) for r in ("double", "float") for a in (3, 4, 5)]
+
+
+def units():
+    """(source, extra flags, object name) for every translation unit."""
+    out = [("qs_sweep_launch.hip", [f"-DQS_SWEEP_R={r}", f"-DQS_SWEEP_A={a}"],
+            f"qs_sweep_launch_{r}_{a}.o") for r, a in SWEEP_VARIANTS]
+    return out + [(s, [], s + ".o") for s in SOURCES]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -35,26 +47,26 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the engine needs ROCm's hipcc to build")
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
-    """Compile each source to an object in parallel, then link the library."""
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile each translation unit to an object in parallel, then link the
+    library."""
     cc = hipcc()
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     objdir = tempfile.mkdtemp(prefix="quaru_hip_build_")
     try:
         procs = []
         objs = []
-        for src in SOURCES:
+        for src, extra, oname in units():
             path = os.path.join(CSRC, src)
-            if not os.path.exists(path):
-                continue
-            obj = os.path.join(objdir, src + ".o")
+            obj = os.path.join(objdir, oname)
             objs.append(obj)
-            cmd = [cc, *HIPCC_FLAGS, "-c", path, "-o", obj]
+            cmd = [cc, *HIPCC_FLAGS, *extra, "-c", path, "-o", obj]
             if verbose:
                 print(" ".join(cmd), flush=True)
             procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,
                                                 stderr=subprocess.STDOUT, text=True)))
-            while len([p for _, p in procs if p.poll() is None]) >= jobs:
-                procs[0][1].wait()
+            while sum(p.poll() is None for _, p in procs) >= jobs:
+                time.sleep(0.1)
         failed = []
         for src, p in procs:
             out, _ = p.communicate()
@@ -81,7 +93,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
 # integration tests and examples, linked against the library above.
 CPP_DIR = os.path.join(HERE, "cpp")
 BIN_DIR = os.path.join(HERE, "bin")
-PROGRAMS = ["register_operation_tests", "examples"]
+PROGRAMS = ["register_operation_tests", "examples", "adjoint_ansatz"]
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 

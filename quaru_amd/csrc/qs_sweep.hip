@@ -26,6 +26,7 @@
 
 #include "qs_sweep.h"
 #include "qs_sweep_kernel.h"
+#include "qs_sweep_launch.h"
 #include "qs_sweep_merge.h"
 
 namespace qs {
@@ -240,36 +241,13 @@ void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, std::vec
   ops.push_back(op);
 }
 
-template <typename R, int A, int W>
-hipError_t launch_sweep(SweepRunner &run, const void *d_blob, uint64_t ntiles) {
-  using V = typename SV2<R>::T;
-  const uint64_t per_block = W == 0 ? kThreads / kWave : 1;
-  const uint64_t grid = (ntiles + per_block - 1) / per_block;
-  const uint32_t xcd_chunk = (sweep_xcd_chunks() && grid % 8 == 0) ? (uint32_t)(grid / 8) : 0u;
-  if (run.n <= 32)
-    hipLaunchKernelGGL((k_sweep<R, A, uint32_t, W>), dim3((unsigned)grid), dim3(kThreads), 0,
-                       run.stream, (V *)run.psi, (const char *)d_blob, xcd_chunk);
-  else
-    hipLaunchKernelGGL((k_sweep<R, A, uint64_t, W>), dim3((unsigned)grid), dim3(kThreads), 0,
-                       run.stream, (V *)run.psi, (const char *)d_blob, xcd_chunk);
-  return hipGetLastError();
-}
-
 template <typename R>
 hipError_t launch_sweep_a(int A, int W, SweepRunner &run, const void *d_blob, uint64_t ntiles) {
-  if (W == kWaveBits) {
-    switch (A) {
-      case 3: return launch_sweep<R, 3, kWaveBits>(run, d_blob, ntiles);
-      case 4: return launch_sweep<R, 4, kWaveBits>(run, d_blob, ntiles);
-      case 5: return launch_sweep<R, 5, kWaveBits>(run, d_blob, ntiles);
-      default: return hipErrorInvalidValue;
-    }
-  }
-  if (W != 0) return hipErrorInvalidValue;
+  const bool xcd = sweep_xcd_chunks();
   switch (A) {
-    case 3: return launch_sweep<R, 3, 0>(run, d_blob, ntiles);
-    case 4: return launch_sweep<R, 4, 0>(run, d_blob, ntiles);
-    case 5: return launch_sweep<R, 5, 0>(run, d_blob, ntiles);
+    case 3: return launch_sweep_ra<R, 3>(W, run.n, run.stream, run.psi, d_blob, ntiles, xcd);
+    case 4: return launch_sweep_ra<R, 4>(W, run.n, run.stream, run.psi, d_blob, ntiles, xcd);
+    case 5: return launch_sweep_ra<R, 5>(W, run.n, run.stream, run.psi, d_blob, ntiles, xcd);
     default: return hipErrorInvalidValue;
   }
 }

@@ -69,6 +69,11 @@ inline void mul2(const cd *a, const cd *b, cd *c) {
   for (int k = 0; k < 4; ++k) c[k] = r[k];
 }
 
+// True when a row-major 2x2 has no off-diagonal entries (exact zeros).
+inline bool diagonal2(const cd *m) {
+  return m[1].re == 0 && m[1].im == 0 && m[2].re == 0 && m[2].im == 0;
+}
+
 // A 2x2 on one qubit with no pattern.
 inline bool plain_2x2(const Gate &g, cd *u) {
   if (g.na != 1 || g.cmask != 0) return false;
@@ -165,6 +170,8 @@ inline void merge_sweep_ops(std::vector<SweepOp> &items, const int *slot, uint64
           if (J.t == t) {  // u moves back to j
             mul2(u, J.m0, J.m0);
             mul2(u, J.m1, J.m1);
+            // the op at j may have been diagonal on t; with u it may not be
+            if (!diagonal2(u)) lastx[t] = j;
             merged = true;
           }
         } else if (slot[t] >= 0 && plain_2x2(*J.g, uj)) {
@@ -174,6 +181,7 @@ inline void merge_sweep_ops(std::vector<SweepOp> &items, const int *slot, uint64
           J.pm = J.pv = 0;
           mul2(u, uj, J.m1);
           for (int k = 0; k < 4; ++k) J.m0[k] = J.m1[k];
+          if (!diagonal2(J.m1)) lastx[t] = j;
           merged = true;
         } else {
           for (int choice = 0; choice < 8; ++choice) {

@@ -313,15 +313,22 @@ class Register:
         """``Register::apply_circuit`` (src/register.rs:345-366), kept on the
         device: the measurements are decided on the GPU (same uniforms, same
         rule as ``measure``), and each conditional operation is applied as an
-        operation controlled on its condition's measured qubits.  That is
-        exact because a circuit's conditional operations never act on a
-        measured qubit (src/operation.rs:302-321): after the collapse,
-        "target measured v" holds exactly where qubit ``target`` is v.  One
-        synchronisation at the end reads the outcomes."""
+        operation controlled on its condition's measured qubits.  After the
+        collapse, "target measured v" holds exactly where qubit ``target`` is
+        v, as long as no conditional operation acts on a measured qubit.
+        ``add_conditional_operation`` only refuses targets measured *before*
+        it (src/operation.rs:302-321); a later ``add_measurement`` may name a
+        conditional operation's target, and then a condition on that qubit
+        must read the recorded outcome, not the qubit: such circuits (and
+        those with more measurements than device slots) are decided on the
+        host.  One synchronisation at the end reads the outcomes."""
         for op in circuit.get_operations():
             self.apply(op)
         targets = list(circuit.get_measurement_targets())
-        if len(targets) > _lib.QS_MEASURE_SLOTS:
+        measured = set(int(t) for t in targets)
+        if len(targets) > _lib.QS_MEASURE_SLOTS or any(
+                measured.intersection(int(t) for t in op.targets())
+                for op, _ in circuit.get_conditional_operations()):
             return self._apply_circuit_host(circuit)
         lib = _lib.load()
         for slot, t in enumerate(targets):
@@ -343,7 +350,7 @@ class Register:
         return [(int(t), bool(out[j])) for j, t in enumerate(targets)]
 
     def _apply_circuit_host(self, circuit: QuantumCircuit) -> List[Tuple[int, bool]]:
-        # more measurements than device slots: measure and decide on the host
+        # measure and decide on the host (src/register.rs:345-366 as written)
         measures: List[Tuple[int, bool]] = []
         if circuit.get_measurement_targets():
             measures = self.measure_circuit(circuit)

@@ -86,11 +86,124 @@ static void unitary2(std::mt19937_64 &rng, cd *m) {
   m[3] = cd_of(d);
 }
 
+// Applies `gates` one by one and as the merged op list; returns the largest
+// amplitude difference, or -1 when the merged list is not encodable.
+static double run_case(const std::vector<Gate> &gates, int n, const int *slot, uint64_t regq,
+                       const std::vector<C> &psi, long *merged_count, int trial) {
+  std::vector<C> ref = psi, got = psi;
+  for (const Gate &g : gates) apply_gate(ref, n, g);
+  std::vector<qs::merge::SweepOp> items(gates.size());
+  for (size_t k = 0; k < gates.size(); ++k) items[k].g = &gates[k];
+  qs::merge::merge_sweep_ops(items, slot, regq);
+  for (const auto &o : items) {
+    if (!o.alive) {
+      ++*merged_count;
+      continue;
+    }
+    if (o.g) {
+      apply_gate(got, n, *o.g);
+    } else {
+      if (o.t < 0 || slot[o.t] < 0) {
+        std::printf("trial %d: merged op on qubit %d, which is not in the tile\n", trial, o.t);
+        return -1;
+      }
+      if (o.pm != 0 && !qs::merge::mux_encodable(o.t, o.pm, slot, regq)) {
+        std::printf("trial %d: merged op has no kernel form (t=%d pm=%llx)\n", trial, o.t,
+                    (unsigned long long)o.pm);
+        return -1;
+      }
+      if (o.pm & (1ull << o.t)) {
+        std::printf("trial %d: pattern holds the target\n", trial);
+        return -1;
+      }
+      apply_mux(got, n, o);
+    }
+  }
+  double d = 0;
+  for (size_t i = 0; i < psi.size(); ++i) d = std::max(d, std::abs(ref[i] - got[i]));
+  return d;
+}
+
+static std::vector<C> random_state(std::mt19937_64 &rng, int n) {
+  std::normal_distribution<double> nd;
+  std::vector<C> psi(1ull << n);
+  double nrm = 0;
+  for (C &v : psi) {
+    v = C(nd(rng), nd(rng));
+    nrm += std::norm(v);
+  }
+  for (C &v : psi) v /= std::sqrt(nrm);
+  return psi;
+}
+
+static Gate dense1(int t, const cd *m, uint64_t cmask = 0, uint64_t cval = 0) {
+  Gate g;
+  g.kind = Gate::kDense;
+  g.na = 1;
+  g.act[0] = t;
+  g.m.assign(m, m + 4);
+  g.cmask = cmask;
+  g.cval = cval;
+  return g;
+}
+
+static Gate diag1(int t, cd a, cd b) {
+  Gate g;
+  g.kind = Gate::kDiag;
+  g.na = 1;
+  g.act[0] = t;
+  g.m = {a, b};
+  return g;
+}
+
+// Fixed sequences that once merged wrongly.  U(3); CX(4->3); RZ(4); H(4);
+// CX(4->3): RZ and H merge into one plain op on 4 that is no longer diagonal,
+// so the second CX must not fold back into the first one across it.  The
+// same with the plain op on 4 built from two diagonal 2x2s and then a dense
+// one (the merged-plain branch), and with a CZ-type phase pattern.
+static int fixed_cases(std::mt19937_64 &rng) {
+  const int n = 9;
+  const cd X[4] = {cd{0, 0}, cd{1, 0}, cd{1, 0}, cd{0, 0}};
+  const double r = 0.7071067811865476;
+  const cd H[4] = {cd{r, 0}, cd{r, 0}, cd{r, 0}, cd{-r, 0}};
+  cd U[4];
+  unitary2(rng, U);
+  const cd e0 = cd_of(std::polar(1.0, -0.4)), e1 = cd_of(std::polar(1.0, 0.4));
+  std::vector<std::vector<Gate>> cases;
+  cases.push_back({dense1(3, U), dense1(3, X, 1ull << 4, 1ull << 4), diag1(4, e0, e1),
+                   dense1(4, H), dense1(3, X, 1ull << 4, 1ull << 4)});
+  cases.push_back({dense1(3, U), dense1(3, X, 1ull << 4, 1ull << 4), diag1(4, e0, e1),
+                   diag1(4, e1, e0), dense1(4, H), dense1(3, X, 1ull << 4, 1ull << 4)});
+  cases.push_back({dense1(3, U), dense1(3, X, 1ull << 4, 0), diag1(4, e0, e1), dense1(4, U),
+                   dense1(3, X, 1ull << 4, 0)});
+  // layouts: both on register slots, the target on a lane bit, both on lanes
+  const int layouts[3][2] = {{0, 1}, {qs::merge::kFirstLaneSlot, 0},
+                             {qs::merge::kFirstLaneSlot, qs::merge::kFirstLaneSlot + 1}};
+  int failures = 0;
+  for (size_t c = 0; c < cases.size(); ++c)
+    for (const auto &lay : layouts) {
+      int slot[64];
+      for (int q = 0; q < 64; ++q) slot[q] = -1;
+      slot[3] = lay[0];
+      slot[4] = lay[1];
+      uint64_t regq = 0;
+      for (int q : {3, 4})
+        if (slot[q] < qs::merge::kFirstLaneSlot) regq |= 1ull << q;
+      long merged = 0;
+      const double d = run_case(cases[c], n, slot, regq, random_state(rng, n), &merged, -1);
+      if (d < 0 || d > 1e-12) {
+        std::printf("fixed case %zu (slots %d,%d): max diff %.3e\n", c, lay[0], lay[1], d);
+        ++failures;
+      }
+    }
+  return failures;
+}
+
 int main() {
   const int n = 9;  // qubits 0..7 in the tile, qubit 8 outside it
   std::mt19937_64 rng(12345);
-  std::normal_distribution<double> nd;
   std::uniform_real_distribution<double> ud(0.0, 6.283185307179586);
+  if (fixed_cases(rng)) return 1;
   long merged_total = 0;
   const int trials = 4000;
   for (int trial = 0; trial < trials; ++trial) {
@@ -193,44 +306,55 @@ int main() {
         for (int k = 0; k < 4; ++k) g.m.push_back(cd_of(std::polar(1.0, ud(rng))));
       }
     }
-    std::vector<C> psi(1ull << n);
-    double nrm = 0;
-    for (C &v : psi) {
-      v = C(nd(rng), nd(rng));
-      nrm += std::norm(v);
-    }
-    for (C &v : psi) v /= std::sqrt(nrm);
-    std::vector<C> ref = psi, got = psi;
-    for (const Gate &g : gates) apply_gate(ref, n, g);
-    std::vector<qs::merge::SweepOp> items(gates.size());
-    for (size_t k = 0; k < gates.size(); ++k) items[k].g = &gates[k];
-    qs::merge::merge_sweep_ops(items, slot, regq);
-    for (const auto &o : items) {
-      if (!o.alive) {
-        ++merged_total;
-        continue;
+    // Dense trials follow some plain diagonal 2x2s with a dense 2x2 on the
+    // same qubit: the two merge into one plain op that is no longer diagonal.
+    if (dense_trial)
+      for (size_t k = 0; k + 1 < gates.size(); ++k) {
+        const Gate &g = gates[k];
+        if (g.kind == Gate::kDiag && g.na == 1 && g.cmask == 0 && slot[g.act[0]] >= 0 &&
+            (rng() & 1)) {
+          cd m[4];
+          unitary2(rng, m);
+          gates[k + 1] = dense1(g.act[0], m);
+        }
       }
-      if (o.g) {
-        apply_gate(got, n, *o.g);
-      } else {
-        if (o.t < 0 || slot[o.t] < 0) {
-          std::printf("trial %d: merged op on qubit %d, which is not in the tile\n", trial, o.t);
-          return 1;
+    // Every fourth trial instead draws from a small menu of gates on three
+    // tile qubits a, b, c with fixed patterns: chains of plain ops on a
+    // pattern qubit between two ops of the same pattern come up often.
+    if (trial % 4 == 3) {
+      const int a = perm[0], b = perm[1], c = perm[2];
+      const cd X[4] = {cd{0, 0}, cd{1, 0}, cd{1, 0}, cd{0, 0}};
+      const uint64_t vb = (fixed_val & 1) << b, vc = ((fixed_val >> 1) & 1) << c;
+      for (Gate &g : gates) {
+        cd m[4];
+        unitary2(rng, m);
+        const int pick = (int)(rng() % 7);
+        const int qa = (rng() & 1) ? a : b;
+        switch (pick) {
+          case 0: g = dense1(qa, m); break;
+          case 1: {
+            const int abc[3] = {a, b, c};
+            g = diag1(abc[rng() % 3], cd_of(std::polar(1.0, ud(rng))),
+                      cd_of(std::polar(1.0, ud(rng))));
+            break;
+          }
+          case 2: g = dense1(a, X, 1ull << b, vb); break;
+          case 3: g = dense1(b, m, 1ull << c, vc); break;
+          case 4:
+            g = Gate();
+            g.kind = Gate::kSparse;
+            g.smask = (1ull << a) | (1ull << b);
+            g.spat = {g.smask};
+            g.m = {cd{-1, 0}};
+            break;
+          case 5: g = dense1(c, m); break;
+          default: g = dense1(a, X, (1ull << b) | (1ull << c), vb | vc); break;
         }
-        if (o.pm != 0 && !qs::merge::mux_encodable(o.t, o.pm, slot, regq)) {
-          std::printf("trial %d: merged op has no kernel form (t=%d pm=%llx)\n", trial, o.t,
-                      (unsigned long long)o.pm);
-          return 1;
-        }
-        if (o.pm & (1ull << o.t)) {
-          std::printf("trial %d: pattern holds the target\n", trial);
-          return 1;
-        }
-        apply_mux(got, n, o);
       }
     }
-    double d = 0;
-    for (size_t i = 0; i < psi.size(); ++i) d = std::max(d, std::abs(ref[i] - got[i]));
+    const std::vector<C> psi = random_state(rng, n);
+    const double d = run_case(gates, n, slot, regq, psi, &merged_total, trial);
+    if (d < 0) return 1;
     if (d > 1e-12) {
       std::printf("trial %d: max diff %.3e over %d gates\n", trial, d, ngates);
       return 1;

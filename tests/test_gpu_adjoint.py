@@ -1,6 +1,7 @@
 """GPU adjoint gradients against parameter shift (CPU oracle / numpy
 goldens), central finite differences on the GPU, and closed forms."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -126,3 +127,139 @@ def test_torch_autograd_function():
     ex2.forward(th.detach().numpy())
     ref = ex2.backward() * w.numpy()[:, None]
     assert np.max(np.abs(th.grad.numpy() - ref)) < 1e-12
+
+
+def test_torch_two_forwards_before_one_backward():
+    """loss = f(t1) + 2 f(t2) through one Expectation: each application keeps
+    its own device state, so both gradients are right; a 1-D thetas gets a
+    gradient of its own shape."""
+    torch = pytest.importorskip("torch")
+    n, layers = 5, 2
+    ans = circuits.hardware_efficient_ansatz(n, layers)
+    masks, coeffs = circuits.zz_chain_observable(n)
+    ex = Expectation(ParamCircuit.from_ansatz(ans), masks, coeffs)
+    t_all = circuits.seeded_thetas(2, ans.nparams, seed=21)
+    t1 = torch.tensor(t_all[0], requires_grad=True)          # 1-D
+    t2 = torch.tensor(t_all[1:2], requires_grad=True)        # (1, P)
+    f = expectation_function()
+    loss = f.apply(t1, ex).sum() + 2.0 * f.apply(t2, ex).sum()
+    loss.backward()
+    assert t1.grad.shape == t1.shape and t2.grad.shape == t2.shape
+    ref = Expectation(ParamCircuit.from_ansatz(ans), masks, coeffs)
+    ref.forward(t_all)
+    g = ref.backward()
+    assert np.max(np.abs(t1.grad.numpy() - g[0])) < 1e-12
+    assert np.max(np.abs(t2.grad.numpy()[0] - 2.0 * g[1])) < 1e-12
+
+
+def _omp_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def oracle_layer_states(ans, theta):
+    """The oracle's state at the start of every layer (and at the end), by
+    the multi-core try_apply_fast restatement (bit-identical to the scalar
+    one)."""
+    n = ans.n
+    s = oracle.new([False] * n)
+    scratch = np.empty_like(s)
+    states = [s.copy()]
+    start = 0
+    for end in ans.layer_ends:
+        for op in ans.ops[start:end]:
+            g = _ansatz_gate(op, theta)
+            oracle.apply_fast_omp(s, scratch, g.matrix(), g.targets(), _omp_threads())
+        states.append(s.copy())
+        start = end
+    return states
+
+
+def _ansatz_gate(op, theta):
+    if op[0] == "ry":
+        return ops.ry(theta[op[2]], op[1])
+    if op[0] == "rz":
+        return ops.rz(theta[op[2]], op[1])
+    return ops.cz(op[1], op[2])
+
+
+def oracle_energy_from(ans, theta, state, layer, masks, coeffs):
+    """Finish the forward pass on the oracle from the start of `layer`."""
+    s = state.copy()
+    scratch = np.empty_like(s)
+    start = 0 if layer == 0 else ans.layer_ends[layer - 1]
+    for op in ans.ops[start:]:
+        g = _ansatz_gate(op, theta)
+        oracle.apply_fast_omp(s, scratch, g.matrix(), g.targets(), _omp_threads())
+    return oracle.expect_zstrings(s, masks, coeffs)
+
+
+_CONFIG5 = {}
+
+
+def _config5_reference():
+    """The differentiation workload at its own size: 20 qubits, 10 layers of
+    RY/RZ on every qubit and a CZ ring (400 parameters), H = sum Z_i + 0.5
+    sum Z_i Z_{i+1}, a batch of 64 seeded parameter sets.  FP64 checks:
+    oracle energies of sets 0 and 1; central finite differences (h = 1e-4)
+    of 16 seeded parameters of set 1 on the complex128 engine; oracle
+    parameter shift of 4 parameters of set 1 in layers 9, 8, 7 and 5
+    (started from the oracle's cached layer states)."""
+    if _CONFIG5:
+        return _CONFIG5
+    n, layers, batch = 20, 10, 64
+    ans = circuits.hardware_efficient_ansatz(n, layers)
+    masks, coeffs = circuits.zz_chain_observable(n)
+    th = circuits.seeded_thetas(batch, ans.nparams, seed=5)
+    print("config 5: oracle forward of sets 0 and 1", flush=True)
+    e0 = oracle_energy_from(ans, th[0], oracle.new([False] * n), 0, masks, coeffs)
+    states1 = oracle_layer_states(ans, th[1])
+    e1 = oracle.expect_zstrings(states1[-1], masks, coeffs)
+    ps = {}
+    per_layer = 2 * n
+    for layer, k in ((9, 3), (8, n + 7), (7, 11), (5, n + 19)):
+        p = layer * per_layer + k
+        tp, tm = th[1].copy(), th[1].copy()
+        tp[p] += math.pi / 2
+        tm[p] -= math.pi / 2
+        ps[p] = 0.5 * (oracle_energy_from(ans, tp, states1[layer], layer, masks, coeffs)
+                       - oracle_energy_from(ans, tm, states1[layer], layer, masks, coeffs))
+        print(f"config 5: parameter shift of parameter {p} done", flush=True)
+    rng = np.random.default_rng(55)
+    fd_params = np.sort(rng.choice(ans.nparams, 16, replace=False))
+    h = 1e-4
+    plus = np.repeat(th[1:2], fd_params.size, axis=0)
+    minus = plus.copy()
+    plus[np.arange(fd_params.size), fd_params] += h
+    minus[np.arange(fd_params.size), fd_params] -= h
+    ex = Expectation(ParamCircuit.from_ansatz(ans), masks, coeffs, dtype="c128")
+    fd = (ex.energies(plus) - ex.energies(minus)) / (2 * h)
+    _CONFIG5.update(ans=ans, masks=masks, coeffs=coeffs, th=th, e=(e0, e1), ps=ps,
+                    fd_params=fd_params, fd=fd)
+    return _CONFIG5
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_config5_full_size(dtype):
+    """20 qubits x 10 layers x batch 64, checkpoint every 2 layers, in both
+    precisions: energies against the oracle, gradients against central FD
+    (FP64, h = 1e-4) and oracle parameter shift, all within 1e-5 (complex128
+    energies within 1e-10)."""
+    ref = _config5_reference()
+    ans, th = ref["ans"], ref["th"]
+    ex = Expectation(ParamCircuit.from_ansatz(ans), ref["masks"], ref["coeffs"], dtype=dtype,
+                     checkpoint_every=2)
+    print(f"config 5 {dtype}: forward + backward, batch {th.shape[0]}", flush=True)
+    e = ex.forward(th)
+    g = ex.backward()
+    assert e.shape == (64,) and g.shape == (64, 400)
+    assert np.isfinite(g).all() and np.max(np.abs(g)) > 0.1
+    etol = 1e-10 if dtype == "c128" else 1e-5
+    assert abs(e[0] - ref["e"][0]) < etol and abs(e[1] - ref["e"][1]) < etol
+    assert np.max(np.abs(g[1, ref["fd_params"]] - ref["fd"])) < 1e-5
+    for p, v in ref["ps"].items():
+        assert abs(g[1, p] - v) < 1e-5, (p, g[1, p], v)
+    if dtype == "c128":
+        # the same batch without checkpoints gives the same gradients
+        ex0 = Expectation(ParamCircuit.from_ansatz(ans), ref["masks"], ref["coeffs"])
+        ex0.forward(th[:8])
+        assert np.max(np.abs(ex0.backward() - g[:8])) < 1e-10

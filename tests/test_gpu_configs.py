@@ -1,9 +1,11 @@
 """The measured workloads at their full sizes on the GPU.
 
 * 26 qubits, complex128 (1 GiB): the random-circuit construction of the
-  12-qubit golden case.  The first two layers are compared per amplitude
-  with the multi-core oracle (1e-12); all 20 layers, followed by their
-  inverse, must return |0...0> within 1e-12 per amplitude (a mirror circuit).
+  12-qubit golden case.  The first two layers (plus merge chains) are
+  compared per amplitude with the multi-core oracle (1e-12), at 26 qubits
+  and moved onto qubits 4..29 of a 30-qubit register; all 20 layers,
+  followed by their inverse, must return |0...0> within 1e-12 per amplitude
+  (a mirror circuit; the state must first have left |0...0>).
 * 30 qubits (16 GiB complex128 + 8 GiB complex64), 20 layers with 8
   Toffoli/CCZ per layer: the complex64 run against the complex128 run
   (infidelity <= 1e-6) and the complex128 mirror circuit back to |0...0>
@@ -37,20 +39,56 @@ def _zero(n: int, dtype: str = "c128") -> Register:
     return Register.new([False] * n, dtype)
 
 
-def test_config2_26q_prefix_matches_oracle():
+_PREFIX26 = {}
+
+
+def _prefix26():
+    """The first two layers of the 26-qubit random circuit, then merge chains
+    (a pattern qubit changed by a diagonal then a dense 2x2 between two ops
+    of the same pattern), and the multi-core oracle's state after them."""
+    if _PREFIX26:
+        return _PREFIX26["ops"], _PREFIX26["ref"]
+    from tests.test_gpu_register import merge_chain_ops
     n = 26
     w = circuits.random_circuit(n, 2, seed=2026, measure=False)
-    print(f"26q prefix: {len(w.ops)} gates", flush=True)
-    r = _zero(n)
-    for op in w.ops:
-        r.apply(op)
+    opl = list(w.ops) + merge_chain_ops(np.random.default_rng(2626), n, 40)
+    print(f"26q prefix: {len(opl)} gates on the oracle", flush=True)
     ref = np.zeros(1 << n, dtype=np.complex128)
     ref[0] = 1.0
     scratch = np.empty_like(ref)
-    for op in w.ops:
+    for op in opl:
         oracle.apply_fast_omp(ref, scratch, op.matrix(), op.targets(), _threads())
+    _PREFIX26.update(ops=opl, ref=ref)
+    return opl, ref
+
+
+def test_config2_26q_prefix_matches_oracle():
+    n = 26
+    opl, ref = _prefix26()
+    r = _zero(n)
+    for op in opl:
+        r.apply(op)
     got = r.state
     assert float(np.max(np.abs(got - ref))) < TOL
+
+
+def test_30q_high_qubits_match_oracle():
+    """The 26-qubit prefix moved onto qubits 4..29 of a 30-qubit register
+    (fusion on): the amplitudes whose low 4 index bits are zero must equal
+    the oracle's 26-qubit state at 1e-12, and carry the whole norm (all
+    others zero).  Qubits 26..29 thus get an amplitude-level check against
+    the oracle, not only a mirror."""
+    n, shift = 30, 4
+    opl, ref = _prefix26()
+    r = _zero(n)
+    for op in opl:
+        r.apply(ops.Operation(op.matrix(), [t + shift for t in op.targets()]))
+    idx = (np.arange(1 << (n - shift), dtype=np.uint64) << np.uint64(shift))
+    got = r.amplitudes(idx)
+    assert float(np.max(np.abs(got - ref))) < TOL
+    inside = float(np.sum(np.abs(got) ** 2))
+    assert abs(inside - 1.0) < 1e-12 and abs(r.norm_sqr() - inside) < 1e-12
+    assert abs(got[0]) < 0.5  # the state moved away from |0...0>
 
 
 def test_config2_26q_mirror_returns_to_zero():
@@ -61,6 +99,7 @@ def test_config2_26q_mirror_returns_to_zero():
     for op in w.ops:
         r.apply(op)
     assert abs(r.norm_sqr() - 1.0) < 1e-12
+    assert abs(r.amplitudes([0])[0]) < 0.5  # the circuit moved the state
     for op in circuits.inverse(w.ops):
         r.apply(op)
     assert r.equals(_zero(n), TOL)
@@ -130,6 +169,7 @@ def test_config3_30q_c64_vs_c128_and_mirror():
     print(f"30q infidelity c64 vs c128: {inf:.3e}", flush=True)
     assert inf <= 1e-6
     del r64
+    assert abs(r128.amplitudes([0])[0]) < 0.5  # the circuit moved the state
     for op in circuits.inverse(w.ops):
         r128.apply(op)
     assert r128.equals(_zero(n), TOL)

@@ -231,6 +231,59 @@ def merge_stress_ops(rng, n, count):
     return opl
 
 
+def merge_chain_ops(rng, n, count):
+    """Chains where a pattern qubit p changes between two ops of the same
+    pattern on t: U(t); CX(p->t) (or a controlled 2x2, or CZ(p, t)); a
+    diagonal 2x2 on p followed by a dense 2x2 on p, which merge into one plain
+    op that is no longer diagonal; the same pattern op on t again.  The second
+    pattern op must not fold back into the first across the op on p.  A few
+    random one-qubit gates sit between chains."""
+    opl = []
+    while len(opl) < count:
+        t, p, o = (int(x) for x in rng.permutation(n)[:3])
+        kind = int(rng.integers(0, 3))
+
+        def pat():
+            if kind == 0:
+                return ops.cnot(p, t)
+            if kind == 1:
+                return ops.cz(p, t)
+            return ops.cnx([p, o], t)
+        opl.append(ops.Operation(random_unitary(rng, 2), [t]))
+        opl.append(pat())
+        opl.append(ops.rz(float(rng.uniform(0, 6)), p))
+        if rng.integers(0, 2):
+            opl.append(ops.phase(p))
+        opl.append(ops.Operation(random_unitary(rng, 2), [p]) if rng.integers(0, 2)
+                   else ops.hadamard(p))
+        opl.append(pat())
+        if rng.integers(0, 2):
+            opl.append(ops.Operation(random_unitary(rng, 2), [int(rng.integers(0, n))]))
+    return opl
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [11, 14])
+def test_sweep_merge_chains_match_oracle(n, dtype):
+    """Merge chains across a pattern qubit (merge_chain_ops) give the
+    oracle's result with fusion on."""
+    rng = np.random.default_rng(901 + n)
+    opl = merge_chain_ops(rng, n, 300)
+    psi = random_state(rng, n)
+    ref = oracle_run(psi.copy(), opl)
+    r = Register.new([False] * n, dtype)
+    r.state = psi
+    r.stats(reset=True)
+    for op in opl:
+        r.apply(op)
+    st = r.stats()
+    assert st["merged"] > 20 and st["sweeps"] > 0
+    if dtype == "c128":
+        assert maxdiff(r.state, ref) < TOL
+    else:
+        assert infid(r.state, ref) < 1e-6
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n", [11, 14])
 def test_sweep_merged_ops_match_oracle(n, dtype):
@@ -585,6 +638,33 @@ def test_apply_circuit_on_device_matches_oracle(dtype):
         else:
             assert maxdiff(r.state, ref) < 1e-5
     assert len(outcomes) >= 4
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_apply_circuit_condition_on_a_later_measured_target(dtype):
+    """measure 0; X(1) if 0 measured 1; measure 1; Z(2) if 1 measured 0.  The
+    second measurement is added after a conditional operation on its qubit
+    (src/operation.rs:293-297 allows that), so the second condition reads the
+    recorded outcome of qubit 1, not qubit 1 after the X: the oracle's
+    apply_circuit on the same uniforms decides."""
+    n = 4
+    seen = set()
+    for seed in range(12):
+        c = ops.QuantumCircuit()
+        c.add_operation(ops.hadamard(0))
+        c.add_operation(ops.hadamard(1))
+        c.add_operation(ops.hadamard(2))
+        c.add_measurement(0)
+        c.add_conditional_operation(ops.not_(1), [(0, True)])
+        c.add_measurement(1)
+        c.add_conditional_operation(ops.pauli_z(2), [(1, False)])
+        ref, want = oracle_circuit(n, c, seed)
+        r = Register.new([False] * n, dtype, seed=seed)
+        got = r.apply_circuit(c)
+        assert got == want
+        seen.add(tuple(v for _, v in got))
+        assert maxdiff(r.state, ref) < (TOL if dtype == "c128" else 1e-6)
+    assert len(seen) == 4
 
 
 def test_measure_deferred_equals_measure():

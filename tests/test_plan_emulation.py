@@ -20,7 +20,7 @@ import oracle
 from quaru_amd import circuits
 from quaru_amd import operation as ops
 from tests.conftest import random_state, random_unitary
-from tests.test_gpu_register import gate_zoo, merge_stress_ops
+from tests.test_gpu_register import gate_zoo, merge_chain_ops, merge_stress_ops
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "quaru_amd", "csrc")
@@ -103,6 +103,28 @@ def test_merge_stress_plans(emu, n, dtype):
     ref = oracle_run(psi.copy(), opl)
     assert st["sweeps"] > 0 and st["merged"] > 40
     assert np.max(np.abs(got - ref)) < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_merge_across_a_changed_pattern_qubit(emu, dtype):
+    """U(3); CX(4->3); RZ(4); H(4); CX(4->3) at 14 qubits: RZ and H merge into
+    one plain op on qubit 4 that is no longer diagonal, so the second CX must
+    stay after it.  (Folding it back into the first CX once gave an error of
+    2.7e-2.)  Then random chains of that shape."""
+    n = 14
+    rng = np.random.default_rng(2718)
+    opl = [ops.Operation(random_unitary(rng, 2), [3]), ops.cnot(4, 3), ops.rz(0.8, 4),
+           ops.hadamard(4), ops.cnot(4, 3)]
+    psi = random_state(rng, n)
+    got, st = emu(n, dtype, psi, opl)
+    assert st["sweeps"] == 1 and st["merged"] >= 2, st
+    assert np.max(np.abs(got - oracle_run(psi.copy(), opl))) < TOL[dtype]
+    for n in (11, 14, 16):
+        opl = merge_chain_ops(rng, n, 300)
+        psi = random_state(rng, n)
+        got, st = emu(n, dtype, psi, opl)
+        assert st["merged"] > 20
+        assert np.max(np.abs(got - oracle_run(psi.copy(), opl))) < TOL[dtype]
 
 
 # Dispatch codes of the lane-target ops, as qs_sweep_kernel.h numbers them:
