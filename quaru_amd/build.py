@@ -47,12 +47,30 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the engine needs ROCm's hipcc to build")
 
 
-def build(verbose: bool = False, jobs: int = 0) -> str:
+OBJDIR = os.path.join(HERE, "_build")
+
+
+def _stale(obj: str, src: str) -> bool:
+    """An object is stale when it is missing or older than its source or any
+    header of csrc (headers are shared, so any change rebuilds every unit)."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    deps.append(os.path.join(os.path.dirname(HERE), "include", "quaru_hip.h"))
+    return os.path.getmtime(obj) <= max(os.path.getmtime(d) for d in deps)
+
+
+def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> str:
     """Compile each translation unit to an object in parallel, then link the
-    library."""
+    library.  ``incremental`` keeps objects in quaru_amd/_build and recompiles
+    only stale ones (a developer shortcut; the default rebuilds everything)."""
     cc = hipcc()
     jobs = jobs or max(1, min(8, os.cpu_count() or 1))
-    objdir = tempfile.mkdtemp(prefix="quaru_hip_build_")
+    if incremental:
+        os.makedirs(OBJDIR, exist_ok=True)
+        objdir = OBJDIR
+    else:
+        objdir = tempfile.mkdtemp(prefix="quaru_hip_build_")
     try:
         procs = []
         objs = []
@@ -60,6 +78,8 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
             path = os.path.join(CSRC, src)
             obj = os.path.join(objdir, oname)
             objs.append(obj)
+            if incremental and not _stale(obj, path):
+                continue
             cmd = [cc, *HIPCC_FLAGS, *extra, "-c", path, "-o", obj]
             if verbose:
                 print(" ".join(cmd), flush=True)
@@ -84,7 +104,8 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
         subprocess.run(cmd, check=True)
         os.replace(tmp, LIB)
     finally:
-        shutil.rmtree(objdir, ignore_errors=True)
+        if not incremental:
+            shutil.rmtree(objdir, ignore_errors=True)
     build_programs(verbose=verbose)
     return LIB
 
@@ -123,4 +144,4 @@ def build_programs(verbose: bool = False) -> list:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))
