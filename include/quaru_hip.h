@@ -173,6 +173,20 @@ int qs_set_state(qs_register *r, const double *in);
  * An index >= 2^n -> QS_ERR_ARGUMENT. */
 int qs_get_amplitudes(qs_register *r, const uint64_t *indices, size_t count, double *out);
 
+/* qs_get_amplitudes with no allocation of any kind (no reference
+ * counterpart): for callers that cannot allocate, such as tight serving
+ * loops that read a few amplitudes per step.  Indices and amplitudes are
+ * staged through a fixed 1.5 MiB device array per GPU, held until the
+ * process ends.  At most 65536 indices per call (more -> QS_ERR_ARGUMENT);
+ * the same results as qs_get_amplitudes.  Calls from one process on one GPU
+ * are serialised by a lock, since they share that GPU's array (which keeps
+ * the last call's contents).  Gates still queued by earlier qs_apply calls
+ * run first, as in every readout; call qs_sync beforehand to keep them (and
+ * the state-sized scratch a dense gate on more than 5 qubits allocates on
+ * first use) out of this call.  Synchronises. */
+int qs_get_amplitudes_noalloc(qs_register *r, const uint64_t *indices, size_t count,
+                              double *out);
+
 /* sum_i |psi_i|^2 in FP64. */
 int qs_norm_sqr(qs_register *r, double *out);
 

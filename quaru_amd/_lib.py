@@ -59,6 +59,7 @@ SIGNATURES = {
     "qs_get_state": (_I, [_P, _P]),
     "qs_set_state": (_I, [_P, _P]),
     "qs_get_amplitudes": (_I, [_P, _P, _S, _P]),
+    "qs_get_amplitudes_noalloc": (_I, [_P, _P, _S, _P]),
     "qs_norm_sqr": (_I, [_P, _DP]),
     "qs_probabilities": (_I, [_P, _P]),
     "qs_inner": (_I, [_P, _P, _DP, _DP]),

@@ -204,6 +204,47 @@ hipError_t big_upload(qs_register *r, const void *src, size_t bytes, void **dev)
 }
 
 // ---------------------------------------------------------------------------
+// scratch of the no-allocation readout (qs_get_amplitudes_noalloc)
+//
+// One fixed array per GPU, defined here as a device variable, so that the
+// entry point needs no hipMalloc: up to kNoAllocMaxIndices uint64 indices,
+// then as many complex128 amplitudes (1.5 MiB, held by every GPU the library
+// runs on until the process ends).  On the host, `scratch` names a host-side
+// stand-in, not device memory: the device address of each GPU's copy comes
+// from hipGetSymbolAddress, called once per device with that device current,
+// and only that address is used.  A per-device mutex serialises the calls of
+// this process on one GPU; the array keeps the last call's contents.
+
+constexpr size_t kNoAllocMaxIndices = 1u << 16;
+constexpr size_t kNoAllocBytes = kNoAllocMaxIndices * (sizeof(uint64_t) + 2 * sizeof(double));
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_noalloc_init_mu;
+std::mutex g_noalloc_mu[kNoAllocDevices];
+char *g_noalloc_ptr[kNoAllocDevices] = {};
+
+int noalloc_scratch(int device, char **out) {
+  if (device < 0 || device >= kNoAllocDevices)
+    return fail(QS_ERR_ARGUMENT, (size_t)device, kNoAllocDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_init_mu);
+  if (!g_noalloc_ptr[device]) {
+    int prev = -1;
+    QS_HIP(hipGetDevice(&prev));
+    if (prev != device) QS_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != device) QS_HIP(hipSetDevice(prev));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(QS_ERR_HIP, 0, 0, "hipGetSymbolAddress(scratch) returned NULL");
+    g_noalloc_ptr[device] = static_cast<char *>(p);
+  }
+  *out = g_noalloc_ptr[device];
+  return QS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // validation (src/register.rs:141-153, :418-429; src/operation.rs:60-69)
 
 int validate_targets(size_t n, const size_t *t, size_t k) {
@@ -1207,6 +1248,36 @@ int qs_get_amplitudes(qs_register *r, const uint64_t *indices, size_t count, dou
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "qs_get_amplitudes");
+  return QS_OK;
+}
+
+int qs_get_amplitudes_noalloc(qs_register *r, const uint64_t *indices, size_t count,
+                              double *out) {
+  if (!r || (count && (!indices || !out))) return fail(QS_ERR_ARGUMENT, 0, 0, "NULL argument");
+  if (count > kNoAllocMaxIndices)
+    return fail(QS_ERR_ARGUMENT, count, kNoAllocMaxIndices,
+                "qs_get_amplitudes_noalloc: more than 65536 indices in one call");
+  for (size_t i = 0; i < count; ++i)
+    if (indices[i] >= r->len)
+      return fail(QS_ERR_ARGUMENT, i, (size_t)indices[i],
+                  "qs_get_amplitudes_noalloc: index >= 2^n");
+  int rc = flush(r);
+  if (rc || count == 0) return rc;
+  char *base = nullptr;
+  rc = noalloc_scratch(r->device, &base);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu[r->device]);
+  const size_t ib = count * sizeof(uint64_t), ob = count * 2 * sizeof(double);
+  uint64_t *d_idx = reinterpret_cast<uint64_t *>(base);
+  double *d_out = reinterpret_cast<double *>(base + kNoAllocMaxIndices * sizeof(uint64_t));
+  hipError_t e = hipMemcpyAsync(d_idx, indices, ib, hipMemcpyHostToDevice, r->stream);
+  if (e == hipSuccess)
+    e = by_dtype(r, [&](auto z) {
+      return qs::launch_gather_amps<decltype(z)>(d_out, r->psi, d_idx, count, r->stream);
+    });
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, ob, hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) return hip_fail(e, "qs_get_amplitudes_noalloc");
   return QS_OK;
 }
 
