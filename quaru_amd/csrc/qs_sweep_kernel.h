@@ -319,6 +319,48 @@ __device__ __forceinline__ void lane_pair(float v, bool hi, float &u, float &w) 
   w = __int_as_float(wi);
 }
 
+// Lane bits 4 and 5 by row transposition: half_swap<K>(x, y) swaps the
+// K-bit-set lanes of x with the K-bit-clear lanes of y (gfx950's
+// v_permlane16_swap / v_permlane32_swap of two different registers).
+// Afterwards every lane holds in (x, y) the amplitude pair (bit K clear,
+// bit K set) of register x's values if its own bit K is clear, of register
+// y's if it is set; applying it again to the pair's two outputs hands every
+// lane its own new values of both registers.  Two swaps per 32 bits per two
+// registers, with no per-lane coefficient select: each lane computes both
+// outputs of one pair.
+template <int K>
+__device__ __forceinline__ void half_swap(int &x, int &y) {
+  static_assert(K == 4 || K == 5, "half swaps move lane bits 4 and 5");
+  if constexpr (K == 4) {
+    const auto p = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+    x = p[0];
+    y = p[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+    x = p[0];
+    y = p[1];
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void half_swap(double &x, double &y) {
+  const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+  int xl = (int)(bx & 0xffffffffll), xh = (int)(bx >> 32);
+  int yl = (int)(by & 0xffffffffll), yh = (int)(by >> 32);
+  half_swap<K>(xl, yl);
+  half_swap<K>(xh, yh);
+  x = __longlong_as_double(((long long)xh << 32) | (unsigned int)xl);
+  y = __longlong_as_double(((long long)yh << 32) | (unsigned int)yl);
+}
+
+template <int K>
+__device__ __forceinline__ void half_swap(float &x, float &y) {
+  int xi = __float_as_int(x), yi = __float_as_int(y);
+  half_swap<K>(xi, yi);
+  x = __int_as_float(xi);
+  y = __int_as_float(yi);
+}
+
 template <typename R, int NA>
 struct Amp {
   R re[NA];
@@ -385,6 +427,41 @@ __device__ __forceinline__ void op_1q_reg(Amp<R, 1 << A> &a, const QS_CONST OpRe
   }
 }
 
+// A 2x2 on lane bit K = 4 or 5 by half swaps, register r paired with
+// r | 2^JP.  coef(r, t) gives the pair's matrix (row-major {re, im} in
+// t[0..7]), the same for r and r | 2^JP; on(r, hi) whether the op acts on
+// this lane's pair (register r if bit K is clear, r | 2^JP if set).
+template <int K, int JP, int A, typename R, typename Coef, typename On>
+__device__ __forceinline__ void lane_2x2_swapped(Amp<R, 1 << A> &a, bool hi, Coef &&coef,
+                                                 On &&on) {
+  constexpr int bp = 1 << JP;
+#pragma unroll
+  for (int r = 0; r < (1 << A); ++r) {
+    if ((r >> JP) & 1) continue;
+    R ur = a.re[r], ui = a.im[r], wr = a.re[r | bp], wi = a.im[r | bp];
+    half_swap<K>(ur, wr);
+    half_swap<K>(ui, wi);
+    R t[8];
+    coef(r, t);
+    R xr = 0, xi = 0, yr = 0, yi = 0;
+    cfma(xr, xi, t[0], t[1], ur, ui);
+    cfma(xr, xi, t[2], t[3], wr, wi);
+    cfma(yr, yi, t[4], t[5], ur, ui);
+    cfma(yr, yi, t[6], t[7], wr, wi);
+    const bool act = on(r, hi);
+    xr = act ? xr : ur;
+    xi = act ? xi : ui;
+    yr = act ? yr : wr;
+    yi = act ? yi : wi;
+    half_swap<K>(xr, yr);
+    half_swap<K>(xi, yi);
+    a.re[r] = xr;
+    a.im[r] = xi;
+    a.re[r | bp] = yr;
+    a.im[r | bp] = yi;
+  }
+}
+
 // 2x2 on a lane qubit: a lane with the bit clear holds x and gets the
 // partner's y: out = m00 x + m01 y; a lane with the bit set holds y and
 // computes out = m10 x + m11 y from the partner's x.
@@ -392,6 +469,21 @@ template <bool kCtl, int K, int A, typename R, typename I>
 __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                            const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
   const bool hi = (c.lane >> K) & 1;
+  if constexpr (K >= 4) {
+    Pred<I> p{};
+    if constexpr (kCtl) p = make_pred<A>(op, c, g);
+    lane_2x2_swapped<K, 0, A>(
+        a, hi,
+        [&](int, R *t) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) t[q] = m[q];
+        },
+        [&](int r, bool h) {
+          if constexpr (kCtl) return p.lane_ok && (h ? pred_on<A>(p, g, r | 1) : pred_on<A>(p, g, r));
+          return true;
+        });
+    return;
+  }
   R mv[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) mv[q] = sld(m + q);
@@ -494,6 +586,23 @@ __device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST Op
   // this lane's output row: the coefficient of its own amplitude (s) and of
   // the partner's (p); selects keep the matrix on scalar loads
   const bool on = p.lane_ok;
+  if constexpr (K >= 4) {
+    // both lanes of a pair see the same pattern (it has no bit on K)
+    R t8[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const R v0 = sld(m + q), v1 = sld(m + 8 + q);
+      t8[q] = on ? v1 : v0;
+    }
+    lane_2x2_swapped<K, 0, A>(
+        a, hi,
+        [&](int, R *t) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) t[q] = t8[q];
+        },
+        [](int, bool) { return true; });
+    return;
+  }
   R mv[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) mv[q] = sld(m + q);
@@ -530,6 +639,19 @@ __device__ __forceinline__ void op_mux_lane_reg(Amp<R, 1 << A> &a, const QS_CONS
   const bool v1 = ((I)op.cval & g.gb[JC]) != 0;
   const QS_CONST R *mb1 = v1 ? m + 8 : m;  // matrix where the slot bit is 1
   const QS_CONST R *mb0 = v1 ? m : m + 8;  // where it is 0
+  if constexpr (K >= 4) {
+    // pair registers across a slot other than JC: both share JC's bit, so
+    // the pair's matrix is fixed at compile time
+    lane_2x2_swapped<K, JC == 0 ? 1 : 0, A>(
+        a, hi,
+        [&](int r, R *t) {
+          const QS_CONST R *mb = ((r >> JC) & 1) ? mb1 : mb0;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) t[q] = mb[q];
+        },
+        [](int, bool) { return true; });
+    return;
+  }
   R w0[8], w1[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -679,6 +801,25 @@ __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
                                            const Ctx<I> &c, const Regs<A, I> &g) {
   const bool hi = (c.lane >> K) & 1;
   const Pred<I> p = make_pred<A>(op, c, g);
+  if constexpr (K >= 4) {
+    // (controlled) exchange by half swaps: each lane swaps its pair's two
+    // amplitudes where the pattern holds (registers r, r | 1 paired)
+#pragma unroll
+    for (int r = 0; r < (1 << A); r += 2) {
+      R ur = a.re[r], ui = a.im[r], wr = a.re[r | 1], wi = a.im[r | 1];
+      half_swap<K>(ur, wr);
+      half_swap<K>(ui, wi);
+      const bool on = p.lane_ok && (hi ? pred_on<A>(p, g, r | 1) : pred_on<A>(p, g, r));
+      R xr = on ? wr : ur, xi = on ? wi : ui, yr = on ? ur : wr, yi = on ? ui : wi;
+      half_swap<K>(xr, yr);
+      half_swap<K>(xi, yi);
+      a.re[r] = xr;
+      a.im[r] = xi;
+      a.re[r | 1] = yr;
+      a.im[r | 1] = yi;
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const R yr = lane_xor<K>(a.re[r], hi), yi = lane_xor<K>(a.im[r], hi);
