@@ -12,6 +12,10 @@
 //
 // Reductions (measure, norm, inner product, equality) accumulate in FP64 for
 // both storage precisions, with a fixed grid so results are deterministic.
+#include <algorithm>
+#include <cstdlib>
+#include <type_traits>
+
 #include "qs_internal.h"
 
 namespace qs {
@@ -317,6 +321,151 @@ __global__ __launch_bounds__(kBlock) void k_gatek(typename V2<R>::T *__restrict_
   }
 }
 
+// Complex64 5-qubit blocks on the matrix cores.  Per wave, 64 groups (32
+// amplitudes each) as the real product OUT = W x IN, with W the 64 x 64 real
+// form [[Mr, -Mi], [Mi, Mr]] of the 32 x 32 complex matrix and IN the 64 x 64
+// matrix of the groups' {re, im} components, by v_mfma_f32_32x32x2f32: two
+// column blocks of 32 groups x two output row blocks (re, im) x 32 steps of
+// K = 2 = 128 instructions per 2048 amplitudes.  FP32 matrix issue is twice
+// the non-packed FP32 vector rate (157.3 against 78.6 TFLOP/s on MI355X),
+// which the complex VALU form of k_gatek<float, 5> needs to reach the memory
+// time.
+//
+// Operand layouts of the instruction (one 32 x 32 block): lane l holds
+// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator register v of lane l
+// holds D[(v % 4) + 8 (v / 4) + 4 (l / 32)][l % 32].  Lane l loads row m of
+// group l, so every load instruction reads 64 consecutive groups (the 6
+// lowest index bits are free).  A half swap (v_permlane32_swap) of rows 2t
+// and 2t + 1 then leaves rows 2t (lanes 0..31) and 2t + 1 (lanes 32..63) of
+// groups 0..31 in the first register, of groups 32..63 in the second: the B
+// operands of step t (real parts; step 16 + t: imaginary parts) of the two
+// column blocks.  The same swap of the two column blocks' accumulators gives
+// rows i and i + 4 of all 64 groups, stored 64 consecutive groups at a time.
+// W's A operands (64 per lane, one per instruction) are read once per
+// persistent wave from the row-major matrix.
+typedef float qs_floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ void half32_swap(float &x, float &y) {
+  const auto p = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(y), false,
+                                                  false);
+  x = __int_as_float(p[0]);
+  y = __int_as_float(p[1]);
+}
+
+template <bool kNT>
+__global__ __launch_bounds__(64) void k_gatek5_mfma(float2 *__restrict__ psi, uint64_t nchunks,
+                                                    Fixed fx, Fixed ax,
+                                                    const float *__restrict__ m) {
+  const int lane = threadIdx.x;
+  const int h = lane >> 5, j = lane & 31;
+  float aop0[32], aop1[32];  // W rows 0..31 (re out) and 32..63 (im out)
+#pragma unroll
+  for (int t = 0; t < 32; ++t) {
+    const int c = 2 * t + h;  // W column
+    const int cc = c & 31;
+    const float mr = m[2 * (32 * j + cc)], mi = m[2 * (32 * j + cc) + 1];
+    aop0[t] = c < 32 ? mr : -mi;
+    aop1[t] = c < 32 ? mi : mr;
+  }
+  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const uint64_t base = expand(ch * 64 + (uint64_t)lane, fx);
+    // b[0][t], b[1][t]: B operands of step t for column blocks 0 and 1
+    float b0[32], b1[32];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const float2 x = ld<kNT>(psi + (base | row_offset<5>(2 * t, ax)));
+      const float2 y = ld<kNT>(psi + (base | row_offset<5>(2 * t + 1, ax)));
+      b0[t] = x.x;
+      b1[t] = y.x;
+      b0[16 + t] = x.y;
+      b1[16 + t] = y.y;
+      half32_swap(b0[t], b1[t]);
+      half32_swap(b0[16 + t], b1[16 + t]);
+    }
+    qs_floatx16 d00 = {}, d01 = {}, d10 = {}, d11 = {};  // d[column block][re/im]
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      d00 = __builtin_amdgcn_mfma_f32_32x32x2f32(aop0[t], b0[t], d00, 0, 0, 0);
+      d01 = __builtin_amdgcn_mfma_f32_32x32x2f32(aop1[t], b0[t], d01, 0, 0, 0);
+      d10 = __builtin_amdgcn_mfma_f32_32x32x2f32(aop0[t], b1[t], d10, 0, 0, 0);
+      d11 = __builtin_amdgcn_mfma_f32_32x32x2f32(aop1[t], b1[t], d11, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      // rows i0 = (v % 4) + 8 (v / 4) and i0 + 4 of all 64 groups
+      float r0 = d00[v], r1 = d10[v], i0 = d01[v], i1 = d11[v];
+      half32_swap(r0, r1);
+      half32_swap(i0, i1);
+      const int row = (v & 3) + 8 * (v >> 2);
+      float2 o;
+      o.x = r0;
+      o.y = i0;
+      st<kNT>(psi + (base | row_offset<5>(row, ax)), o);
+      o.x = r1;
+      o.y = i1;
+      st<kNT>(psi + (base | row_offset<5>(row + 4, ax)), o);
+    }
+  }
+}
+
+// The same with 32 groups per wave, for blocks whose lowest active bit is
+// index bit 5 (so 64 consecutive groups would not be consecutive
+// amplitudes): lane l = 32h + j serves group j and loads its rows 2t + h,
+// which are directly the B operands of steps t (real parts) and 16 + t
+// (imaginary parts); rows 2t and 2t + 1 then differ in bit 5, so each load
+// instruction still reads 512 consecutive bytes.  Its outputs are rows
+// (v % 4) + 8 (v / 4) + 4h of D0 (real) and D1 (imaginary).
+template <bool kNT>
+__global__ __launch_bounds__(64) void k_gatek5_mfma32(float2 *__restrict__ psi,
+                                                      uint64_t nchunks, Fixed fx, Fixed ax,
+                                                      const float *__restrict__ m) {
+  const int lane = threadIdx.x;
+  const int h = lane >> 5, j = lane & 31;
+  float aop0[32], aop1[32];
+#pragma unroll
+  for (int t = 0; t < 32; ++t) {
+    const int c = 2 * t + h;
+    const int cc = c & 31;
+    const float mr = m[2 * (32 * j + cc)], mi = m[2 * (32 * j + cc) + 1];
+    aop0[t] = c < 32 ? mr : -mi;
+    aop1[t] = c < 32 ? mi : mr;
+  }
+  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const uint64_t base = expand(ch * 32 + (uint64_t)j, fx);
+    float br[16], bi[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const float2 v = ld<kNT>(psi + (base | row_offset<5>(2 * t + h, ax)));
+      br[t] = v.x;
+      bi[t] = v.y;
+    }
+    qs_floatx16 d0 = {}, d1 = {};
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const float b = t < 16 ? br[t] : bi[t - 16];
+      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(aop0[t], b, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(aop1[t], b, d1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      float2 o;
+      o.x = d0[v];
+      o.y = d1[v];
+      st<kNT>(psi + (base | row_offset<5>((v & 3) + 8 * (v >> 2) + 4 * h, ax)), o);
+    }
+  }
+}
+
+// Whether complex64 5-qubit blocks use k_gatek5_mfma (QS_GATEK_MFMA, default
+// on); the VALU form k_gatek<float, 5, false> otherwise.
+static bool gatek_mfma() {
+  static const bool on = [] {
+    const char *v = std::getenv("QS_GATEK_MFMA");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
 // Bit layout of one staged block of k_gatek_staged: 64 groups x 2^K rows =
 // 2^(6+K) slots.  Slot bit b sits at state index bit pa[b] (the active
 // positions and the block's 6 lowest free positions, ascending), so the
@@ -526,8 +675,7 @@ static void gatek_dispatch(int na, bool real, F &&f) {
   }
 }
 
-// d_mat layout: K in 3..5 -> transposed D x D; K > 5 -> row major D x D;
-// values in precision R, interleaved.  `scratch` (2^n amplitudes) is needed
+// d_mat layout: row-major D x D {re, im} in precision R.  `scratch` (2^n amplitudes) is needed
 // only when K > 5.
 template <typename R>
 hipError_t launch_gate_dense(void *psi_v, int n, const Gate &g, void *d_mat, void *scratch,
@@ -574,6 +722,24 @@ hipError_t launch_gate_dense(void *psi_v, int n, const Gate &g, void *d_mat, voi
       };
       gatek_dispatch(g.na, real, staged);
       return hipGetLastError();
+    }
+    if constexpr (std::is_same<R, float>::value) {
+      // the matrix cores when the lowest 6 (or 5) index bits are free
+      const uint64_t low = fx_mask(fx);
+      if (g.na == 5 && !real && gatek_mfma() && (low & 0x1Full) == 0 && nfreeN % 64 == 0) {
+        const bool wide = (low & 0x20ull) == 0;
+        const uint64_t nchunks = nfreeN / (wide ? 64 : 32);
+        const unsigned mgrid = (unsigned)std::min<uint64_t>(nchunks, 8192);
+        nt_dispatch(big, [&](auto nt) {
+          if (wide)
+            hipLaunchKernelGGL((k_gatek5_mfma<decltype(nt)::value>), dim3(mgrid), dim3(64), 0,
+                               s, (float2 *)psi, nchunks, fx, ax, (const float *)m);
+          else
+            hipLaunchKernelGGL((k_gatek5_mfma32<decltype(nt)::value>), dim3(mgrid), dim3(64), 0,
+                               s, (float2 *)psi, nchunks, fx, ax, (const float *)m);
+        });
+        return hipGetLastError();
+      }
     }
     const unsigned grid = grid_for(nfreeN, kBlock);
     auto direct = [&](auto kk, auto rr) {

@@ -157,6 +157,46 @@ def test_block_gates_low_and_high_targets(dtype, real):
         assert infid(r.state, ref) < 1e-6
 
 
+@pytest.mark.parametrize("n", [12, 14])
+def test_c64_complex_5q_blocks_matrix_cores(n):
+    """Complex 32 x 32 blocks in complex64 with the 5 lowest index bits free
+    run on the FP32 matrix cores (k_gatek5_mfma): targets in any order, with
+    and without controls (above and between the targets), against the oracle
+    (complex64 infidelity <= 1e-6 and per-amplitude error <= 2e-6).  The
+    shor.rs QFT-add is a 5-qubit 32 x 32 known answer: QFT . add(a) . QFT^-1
+    on |b> gives |a + b mod 32>."""
+    rng = np.random.default_rng(40 + n)
+    hi = list(range(5, n))
+    # lowest active bit 5 (32 groups per wave) and >= 6 (64 groups per wave)
+    opl = [ops.Operation(random_unitary(rng, 32), [9, 5, 7, 6, 8]),
+           ops.Operation(random_unitary(rng, 32), [6, 7, 8, 9, 10])]
+    for _ in range(3):
+        t = [int(x) for x in rng.permutation(hi)[:5]]
+        opl.append(ops.Operation(random_unitary(rng, 32), t))
+    t = [int(x) for x in rng.permutation(hi)[:7]]
+    opl.append(ops.to_controlled(ops.Operation(random_unitary(rng, 32), t[:5]), t[5]))
+    opl.append(ops.Operation.controlled([t[5], t[6]], random_unitary(rng, 32), t[:5],
+                                        control_values=[False, True]))
+    psi0 = random_state(rng, n)
+    ref = oracle_run(psi0.copy(), opl)
+    r = Register.new([False] * n, "c64")
+    r.state = psi0
+    for op in opl:
+        r.apply(op)
+    got = r.state
+    assert infid(got, ref) < 1e-6
+    assert maxdiff(got, ref) < 2e-6
+    # the QFT-add known answer (examples/shors.rs:466-503) on qubits 5..9
+    qft = np.array([[np.exp(2j * np.pi * a * b / 32) for b in range(32)] for a in range(32)]) / np.sqrt(32)
+    for a, b in ((3, 7), (17, 30), (31, 31)):
+        add = np.diag([np.exp(2j * np.pi * a * k / 32) for k in range(32)])
+        m = qft.conj().T @ add @ qft
+        reg = Register.from_int(n, b << 5, "c64")
+        reg.apply(ops.Operation(m, [5, 6, 7, 8, 9]))
+        amp = reg.amplitudes([((a + b) % 32) << 5])[0]
+        assert abs(abs(amp) - 1.0) < 1e-5, (a, b, amp)
+
+
 @pytest.mark.parametrize("fusion", [True, False])
 @pytest.mark.parametrize("n", [8, 12, 13])
 def test_gate_zoo_parity_c128(n, fusion):
