@@ -25,12 +25,15 @@
 // complex64, keeping the reverse sweep within the forward's accuracy).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "../../include/quaru_hip.h"
 #include "qs_internal.h"
+#include "qs_sweep.h"
 
 namespace {
 
@@ -91,17 +94,19 @@ __global__ __launch_bounds__(kBlock) void k_htable(double *h, int n,
 }
 
 // Rotation with one 2x2 matrix per batch element (mats[b*4 + e], complex).
+// Work item w is pair p = w mod 2^(n-1) of state b = w >> (n - 1).
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_rot(typename AV2<R>::T *__restrict__ psi, int n,
                                                 int q, const typename AV2<R>::T *__restrict__ mats,
-                                                uint64_t pairs_per_state, uint64_t total_pairs) {
+                                                uint64_t total_pairs) {
   using T = typename AV2<R>::T;
   const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kItems) + threadIdx.x;
+  const uint64_t pmask = (1ull << (n - 1)) - 1ull;
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const uint64_t w = base + (uint64_t)it * kBlock;
     if (w >= total_pairs) break;
-    const uint64_t b = w / pairs_per_state, p = w % pairs_per_state;
+    const uint64_t b = w >> (n - 1), p = w & pmask;
     const uint64_t off = b << n;
     const uint64_t i0 = off | insert0(p, q), i1 = i0 | (1ull << q);
     const T m0 = mats[4 * b], m1 = mats[4 * b + 1], m2 = mats[4 * b + 2], m3 = mats[4 * b + 3];
@@ -288,6 +293,11 @@ struct qs_adjoint {
   double *d_part = nullptr;
   size_t part_count = 0;
   double *d_grad = nullptr;
+  // descriptor ring of the fused forward pass (device + pinned host)
+  char *d_ring = nullptr, *h_ring = nullptr;
+  size_t ring_cap = 0, ring_off = 0;
+  qs::SweepEngine *engine = nullptr;
+  uint64_t sweeps = 0, kernels = 0;
   size_t amp_bytes() const { return dtype == QS_C128 ? 16 : 8; }
   size_t state_bytes() const { return amp_bytes() * (Bp << n); }
 };
@@ -305,8 +315,39 @@ void free_ctx(qs_adjoint *c) {
   if (c->d_gmat) (void)hipFree(c->d_gmat);
   if (c->d_part) (void)hipFree(c->d_part);
   if (c->d_grad) (void)hipFree(c->d_grad);
+  if (c->d_ring) (void)hipFree(c->d_ring);
+  if (c->h_ring) (void)hipHostFree(c->h_ring);
+  if (c->engine) qs::sweep_engine_free(c->engine);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+// Copy host bytes into the descriptor ring; a slot is reused only after the
+// stream has drained.
+hipError_t ring_upload(qs_adjoint *c, const void *src, size_t bytes, void **dev) {
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (need > c->ring_cap) return hipErrorInvalidValue;
+  if (c->ring_off + need > c->ring_cap) {
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    c->ring_off = 0;
+  }
+  char *h = c->h_ring + c->ring_off;
+  char *d = c->d_ring + c->ring_off;
+  std::memcpy(h, src, bytes);
+  c->ring_off += need;
+  *dev = d;
+  return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream);
+}
+
+// Whether the forward pass runs through the fused sweep engine
+// (QS_ADJOINT_SWEEP, default on) or one kernel per operation.
+bool adjoint_sweeps() {
+  static const bool on = [] {
+    const char *v = std::getenv("QS_ADJOINT_SWEEP");
+    return !(v && v[0] == '0');
+  }();
+  return on;
 }
 
 // Apply a classified fixed gate over all Bp states (n + nb index bits).
@@ -326,6 +367,129 @@ hipError_t apply_fixed(qs_adjoint *c, void *buf, const qs::Gate &g, const void *
   return hipSuccess;
 }
 
+#define AJ(expr)                                          \
+  do {                                                    \
+    e = (expr);                                           \
+    if (e != hipSuccess) return qs::set_hip_error(e, #expr); \
+  } while (0)
+
+// The forward pass through the fused sweep engine over n + nb qubits: the
+// batch index is nb extra high qubits, each rotation a batched 2x2 whose
+// matrix the kernel picks by the tile's batch bits (they are never tile
+// qubits), each fixed gate an ordinary gate.  The operations between two
+// checkpoints go to the planner as one list; the state is copied at each
+// checkpoint.  Gates the planner leaves to per-gate kernels run here.
+template <typename R>
+int forward_sweeps(qs_adjoint *c, const double *thetas) {
+  using T = typename AV2<R>::T;
+  const int n = c->n, ntot = c->n + c->nb;
+  const size_t Bp = c->Bp, B = c->B, P = c->P;
+  hipError_t e = hipSuccess;
+  if (!c->d_ring) {
+    c->ring_cap = 8u << 20;
+    AJ(hipMalloc((void **)&c->d_ring, c->ring_cap));
+    AJ(hipHostMalloc((void **)&c->h_ring, c->ring_cap));
+  }
+  if (!c->engine) c->engine = qs::sweep_engine_new();
+  qs::SweepRunner run;
+  run.n = ntot;
+  run.dtype_c64 = c->dtype == QS_C64;
+  run.psi = c->psi;
+  run.stream = c->stream;
+  run.upload = [c](const void *src, size_t bytes, void **dev) {
+    return ring_upload(c, src, bytes, dev);
+  };
+  run.single = [c, n, Bp](const qs::Gate &g) -> hipError_t {
+    ++c->kernels;
+    if (g.bmask) {
+      std::vector<T> mats(4 * Bp);
+      for (size_t k = 0; k < 4 * Bp; ++k) {
+        mats[k].x = (R)g.m[k].re;
+        mats[k].y = (R)g.m[k].im;
+      }
+      void *d = nullptr;
+      hipError_t e2 = ring_upload(c, mats.data(), mats.size() * sizeof(T), &d);
+      if (e2 != hipSuccess) return e2;
+      const uint64_t tp = (1ull << (n - 1)) * Bp;
+      hipLaunchKernelGGL(k_rot<R>, dim3(grid_of(tp, kBlock * kItems, 0xffffffffu)),
+                         dim3(kBlock), 0, c->stream, (T *)c->psi, n, g.act[0], (const T *)d, tp);
+      return hipGetLastError();
+    }
+    void *tab = nullptr;
+    if (g.kind == qs::Gate::kDiag) {
+      std::vector<R> v;
+      for (const qs::cd &x : g.m) {
+        v.push_back((R)x.re);
+        v.push_back((R)x.im);
+      }
+      hipError_t e2 = ring_upload(c, v.data(), v.size() * sizeof(R), &tab);
+      if (e2 != hipSuccess) return e2;
+    }
+    return apply_fixed<R>(c, c->psi, g, tab);
+  };
+  std::vector<qs::Gate> gates;
+  size_t layer = 0;
+  auto flush = [&]() -> int {
+    if (gates.empty()) return QS_OK;
+    hipError_t e2 = qs::sweep_execute(c->engine, run, gates, &c->sweeps, &c->kernels);
+    gates.clear();
+    if (e2 != hipSuccess) return qs::set_hip_error(e2, "adjoint forward sweeps");
+    return QS_OK;
+  };
+  for (size_t i = 0; i < c->circ.ops.size(); ++i) {
+    const POp &op = c->circ.ops[i];
+    if (op.kind == 1) {
+      qs::Gate g;
+      g.kind = qs::Gate::kDense;
+      g.na = 1;
+      g.act[0] = op.target;
+      g.m.resize(4 * Bp);
+      for (size_t b = 0; b < Bp; ++b) {
+        const double th = b < B ? thetas[b * P + op.param] : 0.0;
+        const double cs = std::cos(th / 2.0), sn = std::sin(th / 2.0);
+        qs::cd *m = g.m.data() + 4 * b;
+        if (op.axis == QS_AXIS_X) {
+          m[0] = {cs, 0};
+          m[1] = {0, -sn};
+          m[2] = {0, -sn};
+          m[3] = {cs, 0};
+        } else if (op.axis == QS_AXIS_Y) {
+          m[0] = {cs, 0};
+          m[1] = {-sn, 0};
+          m[2] = {sn, 0};
+          m[3] = {cs, 0};
+        } else {
+          m[0] = {cs, -sn};
+          m[1] = {0, 0};
+          m[2] = {0, 0};
+          m[3] = {cs, sn};
+        }
+      }
+      if (Bp > 1) {
+        g.bshift = n;
+        g.bmask = (uint32_t)(Bp - 1);
+      }
+      gates.push_back(std::move(g));
+    } else if (op.fwd.kind != qs::Gate::kIdentity) {
+      gates.push_back(op.fwd);
+    }
+    while (layer < c->circ.layer_ends.size() && c->circ.layer_ends[layer] == i + 1) {
+      ++layer;
+      if (c->checkpoint_every > 0 && layer % c->checkpoint_every == 0 &&
+          layer < c->circ.layer_ends.size()) {
+        const int rc = flush();
+        if (rc != QS_OK) return rc;
+        void *cp = nullptr;
+        AJ(hipMalloc(&cp, c->state_bytes()));
+        AJ(hipMemcpyAsync(cp, c->psi, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream));
+        c->ckpt.push_back(cp);
+        c->ckpt_at.push_back(i + 1);
+      }
+    }
+  }
+  return flush();
+}
+
 template <typename R>
 int forward_t(qs_adjoint *c, const double *thetas, const uint64_t *masks, const double *coeffs,
               size_t nterms, double *energies) {
@@ -334,11 +498,6 @@ int forward_t(qs_adjoint *c, const double *thetas, const uint64_t *masks, const 
   const size_t Bp = c->Bp, B = c->B, P = c->P;
   const uint64_t total = (uint64_t)Bp << n;
   hipError_t e = hipSuccess;
-#define AJ(expr)                                          \
-  do {                                                    \
-    e = (expr);                                           \
-    if (e != hipSuccess) return qs::set_hip_error(e, #expr); \
-  } while (0)
   AJ(hipMalloc(&c->psi, c->state_bytes()));
   // observable table
   {
@@ -408,14 +567,18 @@ int forward_t(qs_adjoint *c, const double *thetas, const uint64_t *masks, const 
                      c->stream, (T *)c->psi, n, total);
   AJ(hipGetLastError());
   const uint64_t pps = 1ull << (n - 1);
+  if (adjoint_sweeps()) {
+    const int rc = forward_sweeps<R>(c, thetas);
+    if (rc != QS_OK) return rc;
+  }
   size_t layer = 0;
-  for (size_t i = 0; i < c->circ.ops.size(); ++i) {
+  for (size_t i = 0; i < c->circ.ops.size() && !adjoint_sweeps(); ++i) {
     const POp &op = c->circ.ops[i];
     if (op.kind == 1) {
       const T *mats = (const T *)c->d_rot + (size_t)op.rot_slot * Bp * 4;
       const uint64_t tp = pps * Bp;
       hipLaunchKernelGGL(k_rot<R>, dim3(grid_of(tp, kBlock * kItems, 0xffffffffu)), dim3(kBlock),
-                         0, c->stream, (T *)c->psi, n, op.target, mats, pps, tp);
+                         0, c->stream, (T *)c->psi, n, op.target, mats, tp);
       AJ(hipGetLastError());
     } else {
       AJ(apply_fixed<R>(c, c->psi, op.fwd, (const char *)c->d_gmat + c->gmat_off[2 * i]));

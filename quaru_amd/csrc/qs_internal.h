@@ -43,6 +43,12 @@ struct Gate {
   std::vector<uint32_t> perm;
   // Number of logical gates this entry stands for (apply_all counts n).
   int logical = 1;
+  // Batched 2x2 (kDense, na = 1, no controls; the adjoint's rotations): the
+  // state is 2^nb independent states stacked in the index bits from
+  // `bshift` up, and state b uses m[4b .. 4b + 3].  bmask = 2^nb - 1; 0 for
+  // an ordinary gate.
+  int bshift = 0;
+  uint32_t bmask = 0;
 };
 
 // Bit positions held fixed while the kernel enumerates the free index bits:

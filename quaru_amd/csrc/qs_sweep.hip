@@ -72,7 +72,7 @@ void gate_masks(const Gate &g, uint64_t &x, uint64_t &z) {
 }
 
 bool is_x(const Gate &g) {
-  return g.kind == Gate::kDense && g.na == 1 && g.m[0].re == 0 && g.m[0].im == 0 &&
+  return g.kind == Gate::kDense && g.na == 1 && g.bmask == 0 && g.m[0].re == 0 && g.m[0].im == 0 &&
          g.m[1].re == 1 && g.m[1].im == 0 && g.m[2].re == 1 && g.m[2].im == 0 &&
          g.m[3].re == 0 && g.m[3].im == 0;
 }
@@ -202,6 +202,8 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
   if (g.na == 1) {
     op.type = is_x(g) ? OP_X1 : (g.cmask == 0 ? OP_U1 : OP_C1);
     op.s0 = (uint32_t)slot[g.act[0]];
+    op.bshift = (uint32_t)g.bshift;
+    op.bmask = g.bmask;
   } else {
     op.type = OP_DENSE2;
     op.s0 = (uint32_t)slot[g.act[0]];
@@ -261,7 +263,12 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
   const int W = (sweep_wave_bits(run.dtype_c64) && n >= kLaneBits + A + kWaveBits) ? kWaveBits : 0;
   const int T = kLaneBits + A + W;
   const size_t G = gates.size();
-  if (n < T) {
+  // Batched gates keep their batch bits off the tile (the kernel reads the
+  // batch index from the tile base): tile qubits come from [0, nt).
+  int nt = n;
+  for (const Gate &g : gates)
+    if (g.bmask) nt = std::min(nt, g.bshift);
+  if (nt < T) {
     for (const Gate &g : gates) {
       hipError_t e = run.single(g);
       if (e != hipSuccess) return e;
@@ -349,7 +356,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
       auto climb = [&](uint64_t S, Plan &best) {
         for (int it = 0; it < search_iters; ++it) {
           bool improved = false;
-          for (int qi = lowb; qi < n; ++qi) {
+          for (int qi = lowb; qi < nt; ++qi) {
             if ((S >> qi) & 1ull) continue;
             if (popc(S) < cap) {
               const uint64_t S2 = S | (1ull << qi);
@@ -362,7 +369,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
               }
               continue;
             }
-            for (int qo = lowb; qo < n; ++qo) {
+            for (int qo = lowb; qo < nt; ++qo) {
               if (!((S >> qo) & 1ull)) continue;
               const uint64_t S2 = (S & ~(1ull << qo)) | (1ull << qi);
               if (popc(S2 & ts.high) > ts.hcap) continue;
@@ -379,7 +386,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
         }
       };
       climb(pl.other, pl);
-      const int pool_q = n - lowb;
+      const int pool_q = nt - lowb;
       for (int k = 0; k < search_restarts && pool_q > cap; ++k) {
         uint64_t rs = (uint64_t)*sweeps * 0x9E3779B97F4A7C15ull + (uint64_t)(k + 1) * 0xD1B54A32D192ED03ull;
         uint64_t S = 0;
@@ -483,7 +490,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
         waves.push_back(q);
     }
     uint64_t P = low | pl.other;
-    for (int q = lowb; q < n &&
+    for (int q = lowb; q < nt &&
                        ((int)regs.size() < A || (int)lanes.size() < nfree || (int)waves.size() < W);
          ++q) {
       if ((P >> q) & 1ull) continue;

@@ -134,12 +134,15 @@ __host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t
 
 // 64-byte op record; complex values live in a pool of R scalars
 // ({re, im} interleaved, row-major) at pool[moff].
+// A batched 2x2 (bmask != 0) takes its matrix at pool[moff + 8 b] for the
+// tile's batch index b = (tile base >> bshift) & bmask; the batch bits are
+// never tile qubits, so b is uniform over the tile.
 struct OpRec {
   uint32_t type, s0, s1, na;
   uint32_t act[4];
   uint32_t moff, code;
   uint64_t cmask, cval;
-  uint32_t pad1[2];
+  uint32_t bshift, bmask;
 };
 
 struct SweepHeader {
@@ -206,6 +209,7 @@ __device__ __forceinline__ I goff(const Regs<A, I> &g, int r) {
 template <typename I>
 struct Ctx {
   I gbase;  // tile base | this lane's lane-bit and wave-bit offsets (no register bits)
+  I tbase;  // tile base (uniform)
   int lane;
   int wave;  // wave index in the workgroup
 };
@@ -941,6 +945,9 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
   for (int o = 0; o < nops; ++o) {
     const QS_CONST OpRec &op = ops[o];
     const QS_CONST R *m = pool + op.moff;
+    if (op.bmask)  // the tile base is the same for every lane of a wave
+      m += 8 * __builtin_amdgcn_readfirstlane(
+                   (uint32_t)(((uint64_t)c.tbase >> op.bshift) & op.bmask));
     switch (op.code) {
       QS_REG1(CODE_U1, 0, (op_1q_reg<0, false, A>(a, op, m, c, g)))
       QS_REG1(CODE_U1, 1, (op_1q_reg<1, false, A>(a, op, m, c, g)))
@@ -1089,7 +1096,8 @@ __attribute__((amdgpu_waves_per_eu(((A <= 4 || sizeof(R) == 4) && sizeof(I) == 4
   if (tile >= h.ntiles) return;  // whole waves (W = 0) or whole workgroups
   const Regs<A, I> g = reg_bits<A, I>(h);
   Ctx<I> c;
-  c.gbase = (I)tile_base<T>(tile, h) | lane_offset<I>(h, lane) | wave_offset<W, I>(h, wave);
+  c.tbase = (I)tile_base<T>(tile, h);
+  c.gbase = c.tbase | lane_offset<I>(h, lane) | wave_offset<W, I>(h, wave);
   c.wave = wave;
   c.lane = lane;
   Amp<R, NA> a;

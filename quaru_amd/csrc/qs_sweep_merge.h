@@ -76,7 +76,7 @@ inline bool diagonal2(const cd *m) {
 
 // A 2x2 on one qubit with no pattern.
 inline bool plain_2x2(const Gate &g, cd *u) {
-  if (g.na != 1 || g.cmask != 0) return false;
+  if (g.na != 1 || g.cmask != 0 || g.bmask != 0) return false;
   if (g.kind == Gate::kDense) {
     for (int k = 0; k < 4; ++k) u[k] = g.m[k];
     return true;
@@ -94,6 +94,7 @@ inline bool plain_2x2(const Gate &g, cd *u) {
 // elsewhere", with t not in pm; `choice` picks t among a phase pattern's
 // qubits.  Returns false when the gate has no such form.
 inline bool pattern_2x2(const Gate &g, int choice, int &t, uint64_t &pm, uint64_t &pv, cd *mt) {
+  if (g.bmask != 0) return false;
   if ((g.kind == Gate::kDense || g.kind == Gate::kDiag) && g.na == 1 && g.cmask != 0) {
     if (choice != 0) return false;
     t = g.act[0];

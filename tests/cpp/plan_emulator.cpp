@@ -33,6 +33,17 @@ std::string g_err;
 
 void apply_gate_host(std::vector<C> &psi, int n, const Gate &g) {
   const uint64_t N = 1ull << n;
+  if (g.kind == Gate::kDense && g.bmask) {  // batched 2x2: matrix of state b
+    const uint64_t bt = 1ull << g.act[0];
+    for (uint64_t i = 0; i < N; ++i) {
+      if (i & bt) continue;
+      const qs::cd *m = g.m.data() + 4 * ((i >> g.bshift) & g.bmask);
+      const C x = psi[i], y = psi[i | bt];
+      psi[i] = C(m[0].re, m[0].im) * x + C(m[1].re, m[1].im) * y;
+      psi[i | bt] = C(m[2].re, m[2].im) * x + C(m[3].re, m[3].im) * y;
+    }
+    return;
+  }
   if (g.kind == Gate::kDense) {
     const int K = g.na, D = 1 << K;
     uint64_t am = 0;
@@ -197,6 +208,19 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
       const uint64_t bt = 1ull << t;
       if (cm & bt) return fail("pattern holds the target");
       C m0[4], m1[4];
+      if ((kind == OP_U1 || kind == OP_C1) && o.bmask) {
+        // batched 2x2: the batch bits must stay off the tile
+        if (!need_pool(8 * (o.bmask + 1))) return false;
+        if (tmask & ((uint64_t)o.bmask << o.bshift)) return fail("batch bit on the tile");
+        for (uint64_t i = 0; i < N; ++i) {
+          if ((i & bt) || (kind == OP_C1 && (i & cm) != cv)) continue;
+          const uint32_t mo = o.moff + 8 * (uint32_t)((i >> o.bshift) & o.bmask);
+          const C x = psi[i], y = psi[i | bt];
+          psi[i] = val(mo, 0) * x + val(mo, 1) * y;
+          psi[i | bt] = val(mo, 2) * x + val(mo, 3) * y;
+        }
+        continue;
+      }
       if (kind == OP_U1 || kind == OP_C1) {
         if (!need_pool(8)) return false;
         for (int k = 0; k < 4; ++k) m0[k] = m1[k] = val(o.moff, k);
@@ -272,9 +296,12 @@ extern "C" {
 // hist (128 entries, may be NULL) counts the sweep op records per op code.
 // cmasks/cvals (may be NULL): controls of each gate beyond its matrix, as
 // qs_apply_controlled passes them to the classifier.
-int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const uint64_t *targets,
-            const int *ks, const uint64_t *cmasks, const uint64_t *cvals, uint64_t *stats,
-            uint64_t *hist, char *err, int errlen) {
+// emu_run2 also takes batched 2x2s: gate g with bmasks[g] != 0 (and k = 1)
+// carries bmasks[g] + 1 matrices (state b = (index >> bshift) & bmask uses
+// matrix b), as the adjoint's forward pass builds them.
+int emu_run2(int n, int c64, double *psi, int ngates, const double *mats, const uint64_t *targets,
+             const int *ks, const uint64_t *cmasks, const uint64_t *cvals, const uint32_t *bmasks,
+             int bshift, uint64_t *stats, uint64_t *hist, char *err, int errlen) {
   g_err.clear();
   g_hist = hist;
   std::vector<C> st(psi ? 1ull << n : 0);
@@ -284,6 +311,21 @@ int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const u
   for (int g = 0; g < ngates; ++g) {
     const int k = ks[g];
     std::vector<size_t> t(targets + toff, targets + toff + k);
+    const uint32_t bm = bmasks ? bmasks[g] : 0u;
+    if (bm && k == 1) {
+      Gate G;
+      G.kind = Gate::kDense;
+      G.na = 1;
+      G.act[0] = (int)t[0];
+      G.bshift = bshift;
+      G.bmask = bm;
+      G.m.resize(4 * ((size_t)bm + 1));
+      for (size_t e = 0; e < G.m.size(); ++e) G.m[e] = qs::cd{mats[moff + 2 * e], mats[moff + 2 * e + 1]};
+      moff += 8 * ((size_t)bm + 1);
+      toff += 1;
+      gates.push_back(std::move(G));
+      continue;
+    }
     Gate G = qs::classify_matrix(mats + moff, t.data(), (size_t)k, cmasks ? cmasks[g] : 0,
                                  cvals ? cvals[g] : 0);
     if (G.kind != Gate::kIdentity) gates.push_back(std::move(G));
@@ -320,6 +362,13 @@ int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const u
     psi[2 * i + 1] = st[i].imag();
   }
   return 0;
+}
+
+int emu_run(int n, int c64, double *psi, int ngates, const double *mats, const uint64_t *targets,
+            const int *ks, const uint64_t *cmasks, const uint64_t *cvals, uint64_t *stats,
+            uint64_t *hist, char *err, int errlen) {
+  return emu_run2(n, c64, psi, ngates, mats, targets, ks, cmasks, cvals, nullptr, 0, stats, hist,
+                  err, errlen);
 }
 
 }  // extern "C"

@@ -105,6 +105,69 @@ def test_merge_stress_plans(emu, n, dtype):
     assert np.max(np.abs(got - ref)) < TOL[dtype]
 
 
+def test_batched_rotations_plan(tmp_path):
+    """The adjoint's fused forward pass: a batch of 8 states stacked in index
+    bits 13..15 of a 16-qubit register, layers of per-state RY/RZ rotations
+    (batched 2x2s: the kernel picks state b's matrix by the tile's batch
+    bits) and a CZ ring.  The planner must keep the batch bits off the tile
+    and every state must equal its own circuit on the oracle."""
+    out = str(tmp_path / "libplan_emulator.so")
+    subprocess.run([HIPCC, "-x", "hip", "--offload-arch=gfx950", "-std=c++17", "-O1", "-fPIC",
+                    "-shared", "-I", CSRC, SRC, "-L", LIBDIR, "-lquaru_hip",
+                    f"-Wl,-rpath,{LIBDIR}", "-o", out], check=True, capture_output=True,
+                   timeout=600)
+    lib = ctypes.CDLL(out)
+    lib.emu_run2.restype = ctypes.c_int
+    lib.emu_run2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int] + \
+        [ctypes.c_void_p] * 6 + [ctypes.c_int] + [ctypes.c_void_p] * 2 + \
+        [ctypes.c_char_p, ctypes.c_int]
+    n, nb, layers = 13, 3, 3
+    B = 1 << nb
+    rng = np.random.default_rng(13)
+    mats, tg, ks, bms, per_state = [], [], [], [], [[] for _ in range(B)]
+    for _ in range(layers):
+        for axis in ("ry", "rz"):
+            for q in range(n):
+                th = rng.uniform(0, 2 * np.pi, B)
+                ms = [getattr(ops, axis)(float(t), q).matrix() for t in th]
+                mats.append(np.concatenate([m.reshape(-1) for m in ms]))
+                tg.append([q])
+                ks.append(1)
+                bms.append(B - 1)
+                for b in range(B):
+                    per_state[b].append((ms[b], [q]))
+        for q in range(n):
+            g = ops.cz(q, (q + 1) % n)
+            mats.append(g.matrix().reshape(-1))
+            tg.append(g.targets())
+            ks.append(2)
+            bms.append(0)
+            for b in range(B):
+                per_state[b].append((g.matrix(), g.targets()))
+    psi = np.zeros(1 << (n + nb), dtype=np.complex128)
+    for b in range(B):
+        psi[b << n] = 1.0
+    m_all = np.ascontiguousarray(np.concatenate(mats), dtype=np.complex128)
+    t_all = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.uint64) for t in tg]))
+    ks_a = np.ascontiguousarray(ks, dtype=np.int32)
+    bm_a = np.ascontiguousarray(bms, dtype=np.uint32)
+    for dtype in ("c128", "c64"):
+        st = psi.copy()
+        stats = np.zeros(4, dtype=np.uint64)
+        err = ctypes.create_string_buffer(512)
+        rc = lib.emu_run2(n + nb, 1 if dtype == "c64" else 0, st.ctypes.data, len(ks),
+                          m_all.ctypes.data, t_all.ctypes.data, ks_a.ctypes.data, None, None,
+                          bm_a.ctypes.data, n, stats.ctypes.data, None, err, 512)
+        assert rc == 0, err.value.decode()
+        assert stats[0] >= 2 and stats[1] == 0, stats  # sweeps only
+        for b in range(B):
+            ref = np.zeros(1 << n, dtype=np.complex128)
+            ref[0] = 1.0
+            for m, t in per_state[b]:
+                ref = oracle.apply_fast(ref, m, t)
+            assert np.max(np.abs(st[b << n:(b + 1) << n] - ref)) < TOL[dtype], (dtype, b)
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 def test_merge_across_a_changed_pattern_qubit(emu, dtype):
     """U(3); CX(4->3); RZ(4); H(4); CX(4->3) at 14 qubits: RZ and H merge into
