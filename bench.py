@@ -46,6 +46,7 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 CLASSES = ["sweep", "gate1", "gate2", "gatek", "big", "diag", "sparse", "perm"]
 
 from quaru_amd import Register, circuits  # noqa: E402
+from quaru_amd import operation as qops  # noqa: E402
 from quaru_amd import _lib  # noqa: E402
 
 
@@ -132,12 +133,14 @@ def cpu_baseline(w: circuits.Workload, threads: int):
     cn = next(o for o in w.ops if o._label == "cnot")
     tof = next((o for o in w.ops if o._label == "cnx"), cn)
 
-    def scalar(op):
+    def scalar(op, st=None, nq=None):
+        st = state if st is None else st
+        nq = n if nq is None else nq
         m = np.ascontiguousarray(op.matrix())
         t = np.ascontiguousarray(np.asarray(op.targets(), dtype=np.uint64))
         ea, eb = ctypes.c_size_t(0), ctypes.c_size_t(0)
         t0 = time.perf_counter()
-        rc = L.orc_try_apply_fast(state.ctypes.data_as(ctypes.c_void_p), n,
+        rc = L.orc_try_apply_fast(st.ctypes.data_as(ctypes.c_void_p), nq,
                                   m.ctypes.data_as(ctypes.c_void_p), m.shape[0], m.shape[1],
                                   t.ctypes.data_as(ctypes.c_void_p), t.size, ctypes.byref(ea),
                                   ctypes.byref(eb))
@@ -156,19 +159,41 @@ def cpu_baseline(w: circuits.Workload, threads: int):
     del scratch, state
     layer_gates = n + (n - 1) + (8 if tof is not cn else 0)
     layer_omp = n * per["haar"] + (n - 1) * per["cnot"] + (layer_gates - (2 * n - 1)) * per["toffoli"]
+    # config 2's size: 26 qubits, 20 layers = 520 Haar + 500 CNOT gates
+    n26 = 26
+    s26 = np.zeros(1 << n26, dtype=np.complex128)
+    s26[0] = 1.0
+    w26 = circuits.random_circuit(n26, 1, seed=2026, measure=False)
+    h26 = next(o for o in w26.ops if o._label == "haar")
+    c26 = next(o for o in w26.ops if o._label == "cnot")
+    t26 = {}
+    for name, op in (("haar", h26), ("cnot", c26)):
+        t26[name] = scalar(op, s26, n26)
+    del s26
+    t26_circuit = 520 * t26["haar"] + 500 * t26["cnot"]
+    log(f"cpu baseline: 26 qubits, one thread: haar {t26['haar']:.2f} s, "
+        f"cnot {t26['cnot']:.2f} s")
     return {
         "value": 1.0 / t_scalar,
         "unit": "gates/s",
         "cores": 1,
+        "nproc": os.cpu_count(),
         "kind": "port",
         "sample": (f"one Haar 2x2 gate of the {n}-qubit workload through the oracle's "
                    f"try_apply_fast restatement (filter scan, gather, 2x2 matvec, scatter, "
                    f"out-of-place copy), single thread: {t_scalar:.2f} s; CNOT/Toffoli "
                    f"cost the reference 1.1-1.3x a Haar gate, so gates/s is an upper bound"),
         "multicore": {"value": layer_gates / layer_omp, "unit": "gates/s", "cores": threads,
+                      "nproc": os.cpu_count(),
                       "sample": "one Haar, one CNOT, one Toffoli gate (OpenMP variant), "
-                                "weighted by the per-layer gate mix",
+                                f"weighted by the per-layer gate mix, on every core this "
+                                f"process may run on ({threads} of nproc {os.cpu_count()})",
                       "seconds": per},
+        "config2_26q": {"value": 1020.0 / t26_circuit, "unit": "gates/s", "cores": 1,
+                        "sample": "one Haar and one CNOT gate of the 26-qubit random circuit "
+                                  "(the oracle's try_apply_fast, one thread), extrapolated "
+                                  "to its 520 Haar + 500 CNOT gates",
+                        "seconds": t26, "circuit_seconds": t26_circuit},
     }
 
 
@@ -178,15 +203,30 @@ def adjoint_bench(n: int, layers: int, batch: int, fd_params: int, dtype: str):
     masks, coeffs = circuits.zz_chain_observable(n)
     th = circuits.seeded_thetas(batch, ans.nparams, seed=5)
     pc = ParamCircuit.from_ansatz(ans)
-    ex = Expectation(pc, masks, coeffs, dtype=dtype, checkpoint_every=2)
+    ckpt = 2
+    ex = Expectation(pc, masks, coeffs, dtype=dtype, checkpoint_every=ckpt)
     ex.forward(th)
     ex.backward()  # warmup
     device_sync()
     t0 = time.perf_counter()
-    ex.forward(th)
-    g = ex.backward()
+    _, run = ex.record(th)
+    t1 = time.perf_counter()
+    fstats = run.stats()
+    g = run.backward()
     device_sync()
-    t_adj = time.perf_counter() - t0
+    t2 = time.perf_counter()
+    t_adj, t_fwd, t_bwd = t2 - t0, t1 - t0, t2 - t1
+    # algorithmic HBM bytes of the reverse pass (S = the batch of states):
+    # lambda = h psi (2S), per rotation the fused reverse step reads and
+    # writes psi and lambda (4S), per CZ the inverse phase touches a quarter
+    # of psi and of lambda (S), per checkpoint restore a copy (2S)
+    bp = 1 << max(0, (batch - 1).bit_length())
+    S = float(bp << n) * (16 if dtype == "c128" else 8)
+    nrot = sum(1 for op in ans.ops if op[0] in ("rx", "ry", "rz"))
+    ncz = sum(1 for op in ans.ops if op[0] == "cz")
+    restores = max(0, (layers - 1) // ckpt)
+    bwd_bytes = 2 * S + nrot * 4 * S + ncz * S + restores * 2 * S
+    fwd_bytes = fstats["sweeps"] * 2 * S
     # central finite differences (h = 1e-4, FP64) on the same GPU for a sample
     # of parameters: each needs two forward passes of the whole batch.
     h = 1e-4
@@ -205,10 +245,24 @@ def adjoint_bench(n: int, layers: int, batch: int, fd_params: int, dtype: str):
     grads = batch * ans.nparams
     return {
         "workload": f"{n}-qubit RY/RZ + CZ-ring ansatz, {layers} layers, {ans.nparams} params, "
-                    f"batch {batch}, checkpoint every 2 layers",
+                    f"batch {batch}, checkpoint every {ckpt} layers",
         "dtype": dtype,
         "grads_per_s": grads / t_adj,
         "seconds": t_adj,
+        "forward_seconds": t_fwd,
+        "backward_seconds": t_bwd,
+        "forward_sweeps": fstats["sweeps"],
+        "forward_kernels": fstats["kernels"],
+        "roofline": {
+            "bound": "hbm", "kernel": "k_rot_reverse (fused reverse step on psi and lambda), "
+                                      "the reverse pass as a whole",
+            "achieved": bwd_bytes / t_bwd / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": bwd_bytes / t_bwd / HBM_PEAK, "bytes": bwd_bytes,
+            "note": "algorithmic bytes of the reverse pass / its wall time"},
+        "forward_roofline": {
+            "kernel": "k_sweep with batched rotations (fused forward pass)",
+            "achieved": fwd_bytes / t_fwd / 1e9, "unit": "GB/s",
+            "frac": fwd_bytes / t_fwd / HBM_PEAK, "bytes": fwd_bytes},
         "fd_grads_per_s": grads / t_fd,
         "fd_sample": f"{fd_params} parameters x 2 forward passes of the batch, extrapolated "
                      f"linearly to all {ans.nparams}",
@@ -217,21 +271,52 @@ def adjoint_bench(n: int, layers: int, batch: int, fd_params: int, dtype: str):
     }
 
 
+def measure_bench(reg: Register, n: int):
+    """`measure` (src/register.rs:291-342) on an n-qubit complex128 state: the
+    FP64 probability split reads the state once (2^n x 16 B), the collapse
+    reads and writes it (2 x 2^n x 16 B).  Then a full readout
+    (measure_sequence: one histogram pass over the first 12 qubits, a gather
+    of the consistent amplitudes, one collapse).  Wall time of each call
+    (each synchronises)."""
+    amp = 16
+    reg.sync()
+    device_sync()
+    times = []
+    for target in (0, n // 2, n - 1):
+        t0 = time.perf_counter()
+        reg.measure(target, 0.5)
+        times.append(time.perf_counter() - t0)
+    t_m = sum(times) / len(times)
+    byt = 3.0 * (1 << n) * amp
+    t0 = time.perf_counter()
+    reg.measure_sequence(range(n), [0.5] * n)
+    t_seq = time.perf_counter() - t0
+    return {"workload": f"{n}-qubit complex128 state",
+            "measure": {"ms": t_m * 1e3, "achieved": byt / t_m / 1e9, "unit": "GB/s",
+                        "frac": byt / t_m / HBM_PEAK, "bytes": byt,
+                        "kernels": "k_prob_split (FP64 partials) + k_collapse"},
+            "measure_sequence_all_qubits_ms": t_seq * 1e3}
+
+
 def grover_bench(n: int, iterations: int, seed: int, only=None):
     """The Grover workload at n qubits: H on all, then `iterations` x
     [phase flip of the seeded winner, diffusion].  'fused' applies every
-    Hadamard layer as dense 32 x 32 blocks on 5 qubits (the k-qubit block
-    kernel); 'unfused' as n one-qubit gates (the sweep engine).  Each run is
-    checked against the closed form at the winner and at sampled indices."""
+    Hadamard layer as dense 32 x 32 blocks on 5 qubits, run by the k-qubit
+    block kernel (fusion off); 'unfused' as n one-qubit gates (the sweep
+    engine); 'fused_split' passes the same 32 x 32 blocks with fusion on, so
+    the engine applies them as their one-qubit factors through the sweeps.
+    Each run is checked against the closed form at the winner and at sampled
+    indices."""
     res = {"workload": f"Grover, {n} qubits, {iterations} iterations, seed {seed}"}
-    runs = [("c128_fused", "c128", True), ("c128_unfused", "c128", False),
-            ("c64_fused", "c64", True)]
-    for name, dtype, fused in runs:
+    runs = [("c128_fused", "c128", True, False), ("c128_unfused", "c128", False, True),
+            ("c128_fused_split", "c128", True, True), ("c64_fused", "c64", True, False),
+            ("c64_unfused", "c64", False, True)]
+    for name, dtype, fused, fusion in runs:
         if only and name != only:
             continue
         w, opl = circuits.grover_operations(n, iterations, seed=seed, fused=fused)
         amp = 16 if dtype == "c128" else 8
-        r = Register.new([False] * n, dtype)
+        r = Register.new([False] * n, dtype).set_fusion(fusion)
         dt, ms, cnt, st = timed_run(r, opl, 1, 0)
         aw, ao = circuits.grover_closed_form(n, iterations, w)
         rng = np.random.default_rng(1)
@@ -270,13 +355,16 @@ def main() -> int:
     ap.add_argument("--no-unfused", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--grover32", action="store_true",
-                    help="also run the 32-qubit Grover workload (64 GiB complex128, 32 GiB "
+    ap.add_argument("--no-grover32", action="store_true",
+                    help="skip the 32-qubit Grover workload (64 GiB complex128, 32 GiB "
                          "complex64): 5-qubit fused blocks vs one gate per qubit")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-measure", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
+                    help="threads of the multi-core CPU baseline (default: every core this "
+                         "process may run on)")
     ap.add_argument("--profile", nargs="?", const="c128", default=None,
-                    choices=["c128", "c64", "unfused", "grover_fused", "grover_unfused",
-                             "grover_c64"],
+                    choices=["c128", "c64", "unfused", "measure", "adjoint", "grover_fused",
+                             "grover_unfused", "grover_c64", "grover_c64_unfused"],
                     help="short run for rocprofv3: a 1 GiB torch copy (counter "
                          "calibration), then one warmup-free step of one section")
     args = ap.parse_args()
@@ -286,8 +374,8 @@ def main() -> int:
               file=sys.stderr)
         return 2
     sections = {"c128": True, "c64": not args.no_c64, "unfused": not args.no_unfused,
-                "adjoint": not args.no_adjoint, "cpu": not args.no_cpu,
-                "grover32": args.grover32}
+                "measure": not args.no_measure, "adjoint": not args.no_adjoint,
+                "cpu": not args.no_cpu, "grover32": not args.no_grover32}
     if args.profile:
         args.steps, args.warmup = 1, 0
         sections = {k: False for k in sections}
@@ -402,6 +490,17 @@ def main() -> int:
         ru.set_fusion(True)
         if ru is not reg:
             ru.close()
+    # ---- measurement: probability split + collapse, full readout ----
+    if sections["measure"]:
+        rm = reg
+        if rm is None:
+            rm = Register.new([False] * n)
+            rm.apply_all(qops.hadamard(0))
+        out["measure"] = measure_bench(rm, n)
+        log(f"measure: {out['measure']['measure']['ms']:.2f} ms per measure, "
+            f"{out['measure']['measure']['achieved']:.0f} GB/s")
+        if rm is not reg:
+            rm.close()
     if reg is not None:
         reg.close()
         del reg
@@ -416,7 +515,7 @@ def main() -> int:
         only = None
         if args.profile and args.profile.startswith("grover"):
             only = {"grover_fused": "c128_fused", "grover_unfused": "c128_unfused",
-                    "grover_c64": "c64_fused"}[args.profile]
+                    "grover_c64": "c64_fused", "grover_c64_unfused": "c64_unfused"}[args.profile]
         out["grover32"] = grover_bench(32, 12, seed=4, only=only)
 
     if sections["cpu"]:

@@ -274,6 +274,10 @@ int qs_adjoint_forward(const qs_pcircuit *c, const double *thetas, size_t batch,
 /* One reverse sweep: grads (batch x nparams) = dE_b / dtheta_{b,p}. */
 int qs_adjoint_backward(qs_adjoint *ctx, double *grads);
 
+/* Sweeps and per-gate kernels the fused forward pass of *ctx launched (no
+ * reference counterpart; diagnostics). */
+int qs_adjoint_stats(const qs_adjoint *ctx, uint64_t *sweeps, uint64_t *kernels);
+
 void qs_adjoint_free(qs_adjoint *ctx);
 
 #ifdef __cplusplus

@@ -107,6 +107,14 @@ class AdjointRun:
         except Exception:
             pass
 
+    def stats(self) -> dict:
+        """Sweeps and per-gate kernels of this run's fused forward pass."""
+        if self._ctx is None:
+            raise RuntimeError("the run is closed")
+        sw, kk = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(_lib.load().qs_adjoint_stats(self._ctx, ctypes.byref(sw), ctypes.byref(kk)))
+        return {"sweeps": sw.value, "kernels": kk.value}
+
     def backward(self) -> np.ndarray:
         """dE_b / dtheta_{b,p} as a (batch, nparams) array."""
         if self._ctx is None:

@@ -777,6 +777,13 @@ int qs_adjoint_backward(qs_adjoint *ctx, double *grads) {
   return ctx->dtype == QS_C128 ? backward_t<double>(ctx, grads) : backward_t<float>(ctx, grads);
 }
 
+int qs_adjoint_stats(const qs_adjoint *ctx, uint64_t *sweeps, uint64_t *kernels) {
+  if (!ctx || !sweeps || !kernels) return fail_arg("NULL argument");
+  *sweeps = ctx->sweeps;
+  *kernels = ctx->kernels;
+  return QS_OK;
+}
+
 void qs_adjoint_free(qs_adjoint *ctx) { free_ctx(ctx); }
 
 }  // extern "C"

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -786,6 +787,73 @@ int measure_fast(qs_register *r, const size_t *order, size_t count, const double
   return QS_OK;
 }
 
+// A dense gate on several active qubits that is a Kronecker product of
+// 2x2 factors (Hadamard transforms, layers of one-qubit gates that a caller
+// fused into one matrix) is applied as those factors: each is a one-qubit
+// gate that the sweep engine fuses with its neighbours, one pass over the
+// state for many of them, where the dense form costs a pass (and 2^k complex
+// products per amplitude) for each block.  Controls carry over to every
+// factor (C-(A (x) B) = C-A . C-B).  The product must reproduce the matrix
+// to 1e-14 of its largest entry, far inside the engine's 1e-12 bar.
+bool kron_factors(const Gate &g, std::vector<std::array<cd, 4>> &f) {
+  if (g.kind != Gate::kDense || g.na < 2 || g.na > 12) return false;
+  const int k = g.na;
+  const size_t d = (size_t)1 << k;
+  const cd *M = g.m.data();
+  size_t piv = 0;
+  double amax = -1.0;
+  for (size_t i = 0; i < d * d; ++i) {
+    const double a = std::hypot(M[i].re, M[i].im);
+    if (a > amax) {
+      amax = a;
+      piv = i;
+    }
+  }
+  if (!(amax > 0.0)) return false;
+  const size_t r0 = piv / d, c0 = piv % d;
+  const cd p = M[piv];
+  const double pn = p.re * p.re + p.im * p.im;
+  auto cdiv = [&](const cd &v) {  // v / p
+    return cd{(v.re * p.re + v.im * p.im) / pn, (v.im * p.re - v.re * p.im) / pn};
+  };
+  auto cmul = [](const cd &a, const cd &b) {
+    return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  };
+  f.assign(k, {});
+  for (int a = 0; a < k; ++a) {
+    const size_t ba = (size_t)1 << a;
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y) {
+        const size_t r = x ? (r0 | ba) : (r0 & ~ba), c = y ? (c0 | ba) : (c0 & ~ba);
+        f[a][2 * x + y] = cdiv(M[r * d + c]);
+      }
+  }
+  for (int q = 0; q < 4; ++q) f[0][q] = cmul(f[0][q], p);
+  const double tol = 1e-14 * amax;
+  for (size_t r = 0; r < d; ++r)
+    for (size_t c = 0; c < d; ++c) {
+      cd v{1.0, 0.0};
+      for (int a = 0; a < k; ++a) v = cmul(v, f[a][2 * ((r >> a) & 1) + ((c >> a) & 1)]);
+      if (std::hypot(v.re - M[r * d + c].re, v.im - M[r * d + c].im) > tol) return false;
+    }
+  return true;
+}
+
+// Enqueue a classified gate, as its one-qubit factors when it has them and
+// fusion is on (with fusion off every factor would be a pass of its own).
+int enqueue_split(qs_register *r, Gate &&g) {
+  std::vector<std::array<cd, 4>> f;
+  if (!r->fusion || !kron_factors(g, f)) return enqueue(r, std::move(g));
+  int rc = QS_OK;
+  for (int a = 0; a < g.na && rc == QS_OK; ++a) {
+    const size_t t = (size_t)g.act[a];
+    Gate one = classify(reinterpret_cast<const double *>(f[a].data()), &t, 1, g.cmask, g.cval);
+    one.logical = a == 0 ? g.logical : 0;
+    rc = enqueue(r, std::move(one));
+  }
+  return rc;
+}
+
 }  // namespace
 
 qs::Gate qs::classify_matrix(const double *mat, const size_t *targets, size_t k, uint64_t cmask,
@@ -941,7 +1009,7 @@ int qs_apply(qs_register *r, const double *matrix, size_t rows, size_t cols,
   if (rc) return rc;
   rc = validate_targets((size_t)r->n, targets, k);
   if (rc) return rc;
-  return enqueue(r, classify(matrix, targets, k, 0, 0));
+  return enqueue_split(r, classify(matrix, targets, k, 0, 0));
 }
 
 int qs_apply_controlled(qs_register *r, const size_t *controls, const unsigned char *cvals,
@@ -959,7 +1027,7 @@ int qs_apply_controlled(qs_register *r, const size_t *controls, const unsigned c
     cmask |= 1ull << controls[i];
     if (!cvals || cvals[i]) cval |= 1ull << controls[i];
   }
-  return enqueue(r, classify(matrix, targets, k, cmask, cval));
+  return enqueue_split(r, classify(matrix, targets, k, cmask, cval));
 }
 
 int qs_apply_diagonal(qs_register *r, const double *diag, const size_t *targets, size_t k) {

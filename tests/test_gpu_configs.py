@@ -213,6 +213,33 @@ def _grover_closed_form_register(n: int, t: int, w: int, dtype: str) -> Register
     return e
 
 
+def test_separable_blocks_split_into_sweeps():
+    """Fusion on: separable dense blocks (the 5-qubit Hadamard blocks of the
+    fused Grover workload, an 8-qubit hadamard_transform, a controlled
+    product of two 2x2s) are applied as their one-qubit factors through the
+    sweeps, with the oracle's result; an entangling block stays one gate."""
+    from tests.conftest import random_unitary
+    n, t = 14, 3
+    w, opl = circuits.grover_operations(n, t, seed=9, fused=True)
+    rng = np.random.default_rng(3)
+    u, v = random_unitary(rng, 2), random_unitary(rng, 2)
+    opl = opl + [ops.hadamard_transform(list(range(3, 11))),
+                 ops.Operation.controlled([13], np.kron(u, v), [2, 9]),
+                 ops.Operation(random_unitary(rng, 8), [1, 5, 12])]
+    r = _zero(n)
+    r.stats(reset=True)
+    for op in opl:
+        r.apply(op)
+    got = r.state
+    st = r.stats()
+    ref = np.zeros(1 << n, dtype=np.complex128)
+    ref[0] = 1.0
+    for op in opl:
+        ref = oracle.apply_fast(ref, op.matrix(), op.targets())
+    assert float(np.max(np.abs(got - ref))) < TOL
+    assert st["sweeps"] > 0 and st["gate_kernels"] <= 8, st  # the 3-qubit block and a few
+
+
 def test_grover_closed_form_register_small():
     # the device-built comparison state is itself the closed form (n = 10)
     n, t = 10, 4
@@ -230,12 +257,20 @@ def test_grover_closed_form_register_small():
 
 @pytest.mark.parametrize("fused", [True, False])
 def test_config4_32q_grover_c128(fused):
+    """fused: every Hadamard layer as 5-qubit 32 x 32 blocks, run by the block
+    kernel (fusion off: with fusion on the engine would apply these separable
+    blocks as their one-qubit factors through the sweeps, which the second
+    test below checks at 14 qubits); unfused: one gate per qubit through the
+    sweep engine."""
     n, t = 32, 12
     w, opl = circuits.grover_operations(n, t, seed=4, fused=fused)
     print(f"32q grover c128 fused={fused}: {len(opl)} ops, winner {w}", flush=True)
-    r = _zero(n)
+    r = _zero(n).set_fusion(not fused)
+    r.stats(reset=True)
     for op in opl:
         r.apply(op)
+    st = r.stats()
+    assert (st["sweeps"] == 0) if fused else (st["sweeps"] > 0)
     aw, ao = circuits.grover_closed_form(n, t, w)
     rng = np.random.default_rng(3)
     idx = np.concatenate([[w, 0, (1 << n) - 1], rng.integers(0, 1 << n, 64)]).astype(np.uint64)
@@ -249,7 +284,7 @@ def test_config4_32q_grover_c128(fused):
 def test_config4_32q_grover_c64():
     n, t = 32, 12
     w, opl = circuits.grover_operations(n, t, seed=4, fused=True)
-    r = _zero(n, "c64")
+    r = _zero(n, "c64").set_fusion(False)  # the 5-qubit blocks on the block kernel
     for op in opl:
         r.apply(op)
     e = _grover_closed_form_register(n, t, w, "c128")

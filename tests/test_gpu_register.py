@@ -585,6 +585,80 @@ def test_shor_slow_period_finding_register_ops():
     assert maxdiff(r.state, ref) < 1e-11
 
 
+def _shor_u_gate(targets, number, a, i):
+    """examples/shors.rs:79-92: x -> x * a^(2^i) mod N on the targets, built
+    by to_quantum_gate from the same function (so arguments x >= N also map
+    to x * a^(2^i) mod N, as in the reference)."""
+    apm = pow(a, 1 << i, number)
+    return ops.to_quantum_gate(lambda x: (x * apm) % number, list(targets))
+
+
+def _shor_fast(backend, number, a, uniforms):
+    """find_period_fast of examples/shors.rs:285-361 on a register-like
+    backend: a control qubit reused 2n times (H, controlled U, the R phase
+    from the exponentially weighted sum of earlier outcomes, H, measure,
+    reset on 1).  Returns the measured integer."""
+    n = math.ceil(math.log2(number + 1))
+    backend.apply(ops.not_(1))
+    res, sm = 0, 0.0
+    targets = list(range(1, n + 1))
+    for i in range(2 * n):
+        backend.apply(ops.hadamard(0))
+        backend.apply(ops.to_controlled(_shor_u_gate(targets, number, a, 2 * n - 1 - i), 0))
+        phi = complex(math.cos(-2 * math.pi * sm), math.sin(-2 * math.pi * sm))
+        backend.apply(ops.Operation(np.array([[1, 0], [0, phi]], dtype=np.complex128), [0]))
+        backend.apply(ops.hadamard(0))
+        m = backend.measure(0, float(uniforms[i]))
+        sm /= 2.0
+        if m:
+            backend.apply(ops.not_(0))
+            res |= 1 << i
+            sm += 0.25
+    return res
+
+
+class _OracleRegister:
+    """The oracle with the Register calls _shor_fast makes."""
+
+    def __init__(self, n):
+        self.state = oracle.new([False] * n)
+
+    def apply(self, op):
+        self.state = oracle.apply_fast(self.state, op.matrix(), op.targets())
+        return self
+
+    def measure(self, target, x):
+        rc, r, self.state, _ = oracle.try_measure(self.state, target, x)
+        assert rc == oracle.OK
+        return bool(r)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("number,a", [(15, 7), (21, 2)])
+def test_shor_fast_period_finding_replay(dtype, number, a):
+    """Shor's fast period finder (examples/shors.rs:285-361, a measure-reset
+    loop whose R-gate phase the host computes from earlier outcomes) on the
+    GPU register and on the oracle with the same uniforms: the same outcome
+    bits and the same final state.  Over the seeds the measured phase
+    res / 2^(2n) lands near multiples of 1/r for the true period r."""
+    from fractions import Fraction
+    n = math.ceil(math.log2(number + 1))
+    r_true = next(r for r in range(1, number) if pow(a, r, number) == 1)
+    hits = 0
+    for seed in range(6):
+        rng = circuits.SplitMix64(1000 + seed)
+        u = [rng.uniform() for _ in range(2 * n)]
+        reg = Register.new([False] * (n + 1), dtype)
+        got = _shor_fast(reg, number, a, u)
+        ref = _OracleRegister(n + 1)
+        want = _shor_fast(ref, number, a, u)
+        assert got == want, (seed, got, want)
+        assert maxdiff(reg.state, ref.state) < (TOL if dtype == "c128" else 2e-6)
+        frac = Fraction(got, 1 << (2 * n)).limit_denominator(number - 1)
+        hits += (r_true % frac.denominator == 0)
+    assert hits >= 3
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 def test_permutation_gates_gather(dtype):
     """Permutations with factors on more than 5 qubits run as a gather from a
