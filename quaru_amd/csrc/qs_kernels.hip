@@ -229,6 +229,88 @@ __global__ __launch_bounds__(kBlock) void k_gate1(typename V2<R>::T *__restrict_
   }
 }
 
+// Partner lane l ^ 2^K of a 32-bit value by VALU cross-lane moves: DPP
+// inside rows of 16 lanes for K < 4 -- quad_perm [1,0,3,2] (K = 0),
+// [2,3,0,1] (K = 1), row_shl:4 into banks 0 and 2 plus row_shr:4 into banks
+// 1 and 3 (K = 2: two independent moves, each writing the lanes the other
+// leaves alone), row_ror:8 (K = 3) -- and gfx950's half swaps for K = 4, 5
+// (a value swapped with itself: the first result holds the bit-clear half's
+// values, the second the bit-set half's).
+template <int K>
+__device__ __forceinline__ int lane_partner(int v) {
+  const bool hi = (threadIdx.x >> K) & 1;
+  if constexpr (K == 0) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+  if constexpr (K == 1) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+  if constexpr (K == 2) {
+    const int lo = __builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0x5, false);  // row_shl:4
+    return __builtin_amdgcn_update_dpp(lo, v, 0x114, 0xF, 0xA, false);        // row_shr:4
+  }
+  if constexpr (K == 3) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);
+  if constexpr (K == 4) {
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return hi ? p[0] : p[1];
+  }
+  const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return hi ? p[0] : p[1];
+}
+
+template <int K>
+__device__ __forceinline__ double lane_partner(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_partner<K>((int)(b & 0xffffffffll));
+  const int hi = lane_partner<K>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int K>
+__device__ __forceinline__ float lane_partner(float v) {
+  return __int_as_float(lane_partner<K>(__float_as_int(v)));
+}
+
+// One active target on qubit K < 6, controls (if any) on index bits >= 6:
+// the pairs lie inside runs of 64 consecutive amplitudes, so k_gate1's two
+// loads per pair would each touch every line at part occupancy.  Here lane
+// l of a wave owns amplitude l of a run of 64 consecutive amplitudes (every
+// load and store instruction is one contiguous 1 KiB / 512 B access), gets
+// its partner's amplitude by DPP, and computes its own output with row
+// ((l >> K) & 1) of the matrix.  Chunk c of 64 amplitudes is expand(c, fx)
+// << 6 with fx holding the controls relative to bit 6.
+template <typename R, int K, int kItems, bool kNT>
+__global__ __launch_bounds__(kBlock) void k_gate1_lane(typename V2<R>::T *__restrict__ psi,
+                                                       uint64_t nchunks, Fixed fx,
+                                                       MatArg<R, 2> m) {
+  using T = typename V2<R>::T;
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const bool hi = (lane >> K) & 1;
+  // this lane's row: coefficient of its own amplitude (s) and the partner's (p)
+  const R sr = hi ? m.re[3] : m.re[0], si = hi ? m.im[3] : m.im[0];
+  const R pr = hi ? m.re[2] : m.re[1], pi = hi ? m.im[2] : m.im[1];
+  T a[kItems];
+  uint64_t idx[kItems];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t c = wave * kItems + it;
+    if (c < nchunks) {
+      idx[it] = (expand(c, fx) << 6) | (uint64_t)lane;
+      a[it] = ld<kNT>(psi + idx[it]);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t c = wave * kItems + it;
+    if (c < nchunks) {  // uniform over the wave: whole chunks only
+      T y;
+      y.x = lane_partner<K>(a[it].x);
+      y.y = lane_partner<K>(a[it].y);
+      T r = czero<T>();
+      r = cmla(r, sr, si, a[it]);
+      r = cmla(r, pr, pi, y);
+      st<kNT>(psi + idx[it], r);
+    }
+  }
+}
+
 // Two active targets: quadruples (b, b|t0, b|t1, b|t0|t1) and a 4x4 matrix.
 template <typename R, int kItems, bool kNT>
 __global__ __launch_bounds__(kBlock) void k_gate2(typename V2<R>::T *__restrict__ psi,
@@ -689,7 +771,36 @@ hipError_t launch_gate_dense(void *psi_v, int n, const Gate &g, void *d_mat, voi
   const bool big = streaming(n, sizeof(T));
   // pair partners closer than a 128-byte line: plain loads (see kStreamBytes)
   const auto near = [](int q) { return (sizeof(T) << q) < 128; };
-  if (g.na == 1) {
+  // QS_GATE1_LANE=0 keeps k_gate1 for every target qubit
+  static const bool lane_form = [] {
+    const char *v = std::getenv("QS_GATE1_LANE");
+    return !(v && v[0] == '0');
+  }();
+  if (g.na == 1 && g.act[0] < 6 && n >= 6 && (g.cmask & 0x3Full) == 0 && lane_form) {
+    // measured at 30 qubits, complex128: 5.85 ms per gate on qubits 0..5
+    // against 6.28-6.54 ms for k_gate1; 8 chunks per wave, or two runs 64 KiB
+    // apart per workgroup, ran slower (6.0-6.4 ms)
+    constexpr int kItems = 4;
+    const Fixed fxc = make_fixed(g.cmask >> 6, g.cval >> 6);
+    const uint64_t nchunks = 1ull << (n - 6 - fxc.nf);
+    const unsigned grid = grid_for(nchunks * 64, kBlock * kItems);
+    const MatArg<R, 2> ma = to_matarg<R, 2>(g);
+    nt_dispatch(big, [&](auto nt) {
+      constexpr bool NT = decltype(nt)::value;
+      auto go = [&](auto kk) {
+        hipLaunchKernelGGL((k_gate1_lane<R, decltype(kk)::value, kItems, NT>), dim3(grid),
+                           dim3(kBlock), 0, s, psi, nchunks, fxc, ma);
+      };
+      switch (g.act[0]) {
+        case 0: go(std::integral_constant<int, 0>{}); break;
+        case 1: go(std::integral_constant<int, 1>{}); break;
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 3: go(std::integral_constant<int, 3>{}); break;
+        case 4: go(std::integral_constant<int, 4>{}); break;
+        default: go(std::integral_constant<int, 5>{}); break;
+      }
+    });
+  } else if (g.na == 1) {
     constexpr int kItems = 4;
     nt_dispatch(big && !near(g.act[0]), [&](auto nt) {
       hipLaunchKernelGGL((k_gate1<R, kItems, decltype(nt)::value>),

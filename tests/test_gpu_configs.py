@@ -117,13 +117,17 @@ def _streaming_zoo(rng, n):
         q, r = np.linalg.qr(rng.standard_normal((d, d)))
         return (q * np.sign(np.diag(r))).astype(np.complex128)
 
-    opl = [ops.Operation(random_unitary(rng, 2), [q]) for q in (0, 1, 2, 3, 12, n - 1)]
+    opl = [ops.Operation(random_unitary(rng, 2), [q]) for q in (0, 1, 2, 3, 4, 5, 12, n - 1)]
     opl += [ops.Operation(random_unitary(rng, 4), [1, n - 2]),
             ops.Operation(random_unitary(rng, 4), [7, 20])]
     for k in (3, 4, 5):
         opl += [ops.Operation(random_unitary(rng, 1 << k), list(range(k))),
                 ops.Operation(random_unitary(rng, 1 << k), list(range(n - k, n))),
                 ops.Operation(orth(1 << k), list(range(8, 8 + k)))]
+    # one-qubit gates on qubits 0..2 under controls on bits >= 6 (the lane form)
+    opl += [ops.to_controlled(ops.Operation(random_unitary(rng, 2), [1]), 20),
+            ops.Operation.controlled([7, 18], random_unitary(rng, 2), [2],
+                                     control_values=[True, False])]
     opl += [ops.cnot(0, n - 1), ops.cnx([3, 9], 17), ops.cnz([2, 5, 11], 23), ops.rz(0.7, 14),
             ops.Operation(np.diag(np.exp(1j * rng.uniform(0, 6, 8))), [4, 13, 22]),
             ops.to_controlled(ops.Operation(random_unitary(rng, 2), [n - 3]), 1)]
