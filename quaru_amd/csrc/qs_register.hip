@@ -74,8 +74,10 @@ struct qs_register {
   void *scratch = nullptr;  // 2^n amplitudes, only for dense ops with > 5 active bits
   double *d_red = nullptr;  // reduction partials + results
   size_t d_red_bytes = 0;
-  double *d_big = nullptr;  // measurement work buffer, grown on demand
+  double *d_big = nullptr;  // measurement and readout work buffer, grown on demand
   size_t d_big_bytes = 0;
+  void *d_upload = nullptr;  // tables of gates too large for the ring, grown on demand
+  size_t d_upload_bytes = 0;
   double *h_red = nullptr;  // pinned host mirror of the results
   // matrix upload ring: device + pinned host, slots reused after a sync
   char *d_ring = nullptr;
@@ -106,6 +108,11 @@ struct qs_register {
 namespace {
 
 constexpr size_t kRingBytes = 8u << 20;
+// Readouts (state, probabilities, amplitudes) go through the register's work
+// buffer in pieces of at most 2^22 amplitudes (64 MiB widened to complex128),
+// so they allocate nothing after the first call.
+constexpr int kReadoutBits = 22;
+constexpr uint64_t kReadoutAmps = 1ull << kReadoutBits;
 constexpr size_t kRedDoubles = 4 * qs::kReduceBlocks + 64;
 
 bool valid_dtype(int d) { return d == QS_C128 || d == QS_C64; }
@@ -193,12 +200,23 @@ hipError_t ring_upload(qs_register *r, const void *src, size_t bytes, void **dev
   return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, r->stream);
 }
 
-// Upload a large buffer through a temporary allocation (dense ops with many
-// active qubits).  Synchronises; returns the allocation for the caller to
-// free after the next sync.
+// Upload a table too large for the ring (dense or permutation gates on many
+// active qubits) into the register's upload buffer, grown on demand and
+// kept: the stream is drained before a smaller buffer is replaced, and after
+// the copy, so the host source may go away.
 hipError_t big_upload(qs_register *r, const void *src, size_t bytes, void **dev) {
-  hipError_t e = hipMalloc(dev, bytes);
-  if (e != hipSuccess) return e;
+  hipError_t e = hipSuccess;
+  if (r->d_upload_bytes < bytes) {
+    e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return e;
+    if (r->d_upload) (void)hipFree(r->d_upload);
+    r->d_upload = nullptr;
+    r->d_upload_bytes = 0;
+    e = hipMalloc(&r->d_upload, bytes);
+    if (e != hipSuccess) return e;
+    r->d_upload_bytes = bytes;
+  }
+  *dev = r->d_upload;
   e = hipMemcpyAsync(*dev, src, bytes, hipMemcpyHostToDevice, r->stream);
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(r->stream);
@@ -407,19 +425,12 @@ hipError_t launch_one(qs_register *r, const Gate &g) {
         buf[2 * i + 1] = (T)g.m[i].im;
       }
       void *dm = nullptr;
-      void *tmp = nullptr;
       const size_t bytes = buf.size() * sizeof(T);
       hipError_t e = bytes + 256 <= r->ring_cap ? ring_upload(r, buf.data(), bytes, &dm)
-                                                : big_upload(r, buf.data(), bytes, &tmp);
+                                                : big_upload(r, buf.data(), bytes, &dm);
       if (e != hipSuccess) return e;
-      if (tmp) dm = tmp;
       ++r->st_kernels;
       e = qs::launch_gate_diag<R>(r->psi, r->n, g, dm, r->stream);
-      if (tmp) {
-        hipError_t e2 = hipStreamSynchronize(r->stream);
-        (void)hipFree(tmp);
-        if (e == hipSuccess) e = e2;
-      }
       return e;
     }
     case Gate::kPerm: {
@@ -433,28 +444,20 @@ hipError_t launch_one(qs_register *r, const Gate &g) {
         ph[2 * i + 1] = (T)g.m[i].im;
       }
       void *dm = nullptr;
-      void *tmp = nullptr;
       hipError_t e = buf.size() + 256 <= r->ring_cap ? ring_upload(r, buf.data(), buf.size(), &dm)
-                                                     : big_upload(r, buf.data(), buf.size(), &tmp);
+                                                     : big_upload(r, buf.data(), buf.size(), &dm);
       if (e != hipSuccess) return e;
-      if (tmp) dm = tmp;
       if (!r->scratch) {
         e = hipMalloc(&r->scratch, r->state_bytes());
         if (e != hipSuccess) return e;
       }
       ++r->st_kernels;
       e = qs::launch_gate_perm<R>(r->psi, r->n, g, dm, r->scratch, r->stream);
-      if (tmp) {
-        hipError_t e2 = hipStreamSynchronize(r->stream);
-        (void)hipFree(tmp);
-        if (e == hipSuccess) e = e2;
-      }
       return e;
     }
     case Gate::kDense: {
       const size_t d = (size_t)1 << g.na;
       void *dm = nullptr;
-      void *tmp = nullptr;
       hipError_t e = hipSuccess;
       if (g.na >= 3) {
         // row-major {re, im} in the register's precision
@@ -465,9 +468,8 @@ hipError_t launch_one(qs_register *r, const Gate &g) {
         }
         const size_t bytes = buf.size() * sizeof(T);
         e = bytes + 256 <= r->ring_cap ? ring_upload(r, buf.data(), bytes, &dm)
-                                       : big_upload(r, buf.data(), bytes, &tmp);
+                                       : big_upload(r, buf.data(), bytes, &dm);
         if (e != hipSuccess) return e;
-        if (tmp) dm = tmp;
         if (g.na > 5 && !r->scratch) {
           e = hipMalloc(&r->scratch, r->state_bytes());
           if (e != hipSuccess) return e;
@@ -475,11 +477,6 @@ hipError_t launch_one(qs_register *r, const Gate &g) {
       }
       ++r->st_kernels;
       e = qs::launch_gate_dense<R>(r->psi, r->n, g, dm, r->scratch, r->stream);
-      if (tmp) {
-        hipError_t e2 = hipStreamSynchronize(r->stream);
-        (void)hipFree(tmp);
-        if (e == hipSuccess) e = e2;
-      }
       return e;
     }
   }
@@ -990,6 +987,7 @@ void qs_register_free(qs_register *r) {
   if (r->scratch) (void)hipFree(r->scratch);
   if (r->d_red) (void)hipFree(r->d_red);
   if (r->d_big) (void)hipFree(r->d_big);
+  if (r->d_upload) (void)hipFree(r->d_upload);
   if (r->h_red) (void)hipHostFree(r->h_red);
   if (r->d_ring) (void)hipFree(r->d_ring);
   if (r->h_ring) (void)hipHostFree(r->h_ring);
@@ -1198,7 +1196,7 @@ int qs_sample_cdf(qs_register *r, const double *uniforms, size_t count, uint64_t
   const uint64_t nchunks = (r->len + chunk - 1) / chunk;
   std::vector<double> sums(nchunks);
   double *d_sums = nullptr;
-  QS_HIP(hipMalloc((void **)&d_sums, std::max<uint64_t>(nchunks, chunk) * sizeof(double)));
+  QS_HIP(ensure_big(r, std::max<uint64_t>(nchunks, chunk) * sizeof(double), &d_sums));
   hipError_t e = by_dtype(r, [&](auto z) {
     return qs::launch_chunk_sums<decltype(z)>(r->psi, r->n, chunk, d_sums, r->stream);
   });
@@ -1206,10 +1204,7 @@ int qs_sample_cdf(qs_register *r, const double *uniforms, size_t count, uint64_t
     e = hipMemcpyAsync(sums.data(), d_sums, nchunks * sizeof(double), hipMemcpyDeviceToHost,
                        r->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-  if (e != hipSuccess) {
-    (void)hipFree(d_sums);
-    return hip_fail(e, "qs_sample_cdf");
-  }
+  if (e != hipSuccess) return hip_fail(e, "qs_sample_cdf");
   std::vector<double> prefix(nchunks + 1, 0.0);
   for (uint64_t c = 0; c < nchunks; ++c) prefix[c + 1] = prefix[c] + sums[c];
   const double total = prefix[nchunks];
@@ -1250,7 +1245,6 @@ int qs_sample_cdf(qs_register *r, const double *uniforms, size_t count, uint64_t
     }
     indices[j] = found ? x : last_nz;
   }
-  (void)hipFree(d_sums);
   if (e != hipSuccess) return hip_fail(e, "qs_sample_cdf");
   return QS_OK;
 }
@@ -1264,14 +1258,20 @@ int qs_get_state(qs_register *r, double *out) {
     QS_HIP(hipStreamSynchronize(r->stream));
     return QS_OK;
   }
+  // widened to complex128 through the work buffer, kReadoutAmps at a time
+  const uint64_t step = std::min<uint64_t>(r->len, kReadoutAmps);
   double *d = nullptr;
-  QS_HIP(hipMalloc((void **)&d, r->len * 2 * sizeof(double)));
-  hipError_t e = qs::launch_convert_to_c128<float>(d, r->psi, r->len, r->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(out, d, r->len * 2 * sizeof(double), hipMemcpyDeviceToHost, r->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-  (void)hipFree(d);
-  if (e != hipSuccess) return hip_fail(e, "qs_get_state");
+  QS_HIP(ensure_big(r, step * 2 * sizeof(double), &d));
+  for (uint64_t off = 0; off < r->len; off += step) {
+    const uint64_t cnt = std::min<uint64_t>(step, r->len - off);
+    hipError_t e = qs::launch_convert_to_c128<float>(
+        d, (const char *)r->psi + off * r->amp_bytes(), cnt, r->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out + 2 * off, d, cnt * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                         r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return hip_fail(e, "qs_get_state");
+  }
   return QS_OK;
 }
 
@@ -1284,14 +1284,19 @@ int qs_set_state(qs_register *r, const double *in) {
     QS_HIP(hipStreamSynchronize(r->stream));
     return QS_OK;
   }
+  const uint64_t step = std::min<uint64_t>(r->len, kReadoutAmps);
   double *d = nullptr;
-  QS_HIP(hipMalloc((void **)&d, r->len * 2 * sizeof(double)));
-  hipError_t e = hipMemcpyAsync(d, in, r->len * 2 * sizeof(double), hipMemcpyHostToDevice,
-                                r->stream);
-  if (e == hipSuccess) e = qs::launch_convert_from_c128<float>(r->psi, d, r->len, r->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-  (void)hipFree(d);
-  if (e != hipSuccess) return hip_fail(e, "qs_set_state");
+  QS_HIP(ensure_big(r, step * 2 * sizeof(double), &d));
+  for (uint64_t off = 0; off < r->len; off += step) {
+    const uint64_t cnt = std::min<uint64_t>(step, r->len - off);
+    hipError_t e = hipMemcpyAsync(d, in + 2 * off, cnt * 2 * sizeof(double),
+                                  hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess)
+      e = qs::launch_convert_from_c128<float>((char *)r->psi + off * r->amp_bytes(), d, cnt,
+                                              r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return hip_fail(e, "qs_set_state");
+  }
   return QS_OK;
 }
 
@@ -1302,20 +1307,26 @@ int qs_get_amplitudes(qs_register *r, const uint64_t *indices, size_t count, dou
       return fail(QS_ERR_ARGUMENT, i, (size_t)indices[i], "qs_get_amplitudes: index >= 2^n");
   int rc = flush(r);
   if (rc || count == 0) return rc;
-  char *d = nullptr;
-  const size_t ib = count * sizeof(uint64_t), ob = count * 2 * sizeof(double);
-  QS_HIP(hipMalloc((void **)&d, ib + ob));
+  // indices and amplitudes through the work buffer, `step` at a time
+  const size_t step = std::min<size_t>(count, kReadoutAmps);
+  double *d = nullptr;
+  QS_HIP(ensure_big(r, step * (sizeof(uint64_t) + 2 * sizeof(double)), &d));
   uint64_t *d_idx = (uint64_t *)d;
-  double *d_out = (double *)(d + ib);
-  hipError_t e = hipMemcpyAsync(d_idx, indices, ib, hipMemcpyHostToDevice, r->stream);
-  if (e == hipSuccess)
-    e = by_dtype(r, [&](auto z) {
-      return qs::launch_gather_amps<decltype(z)>(d_out, r->psi, d_idx, count, r->stream);
-    });
-  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, ob, hipMemcpyDeviceToHost, r->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-  (void)hipFree(d);
-  if (e != hipSuccess) return hip_fail(e, "qs_get_amplitudes");
+  double *d_out = d + step;
+  for (size_t off = 0; off < count; off += step) {
+    const size_t cnt = std::min(step, count - off);
+    hipError_t e = hipMemcpyAsync(d_idx, indices + off, cnt * sizeof(uint64_t),
+                                  hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess)
+      e = by_dtype(r, [&](auto z) {
+        return qs::launch_gather_amps<decltype(z)>(d_out, r->psi, d_idx, cnt, r->stream);
+      });
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out + 2 * off, d_out, cnt * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                         r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return hip_fail(e, "qs_get_amplitudes");
+  }
   return QS_OK;
 }
 
@@ -1362,16 +1373,21 @@ int qs_probabilities(qs_register *r, double *out) {
   if (!r || !out) return fail(QS_ERR_ARGUMENT, 0, 0, "NULL argument");
   int rc = flush(r);
   if (rc) return rc;
+  // 2^m amplitudes at a time through the work buffer
+  const int m = std::min(r->n, kReadoutBits);
+  const uint64_t step = 1ull << m;
   double *d = nullptr;
-  QS_HIP(hipMalloc((void **)&d, r->len * sizeof(double)));
-  hipError_t e = by_dtype(r, [&](auto z) {
-    return qs::launch_probabilities<decltype(z)>(r->psi, r->n, d, r->stream);
-  });
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(out, d, r->len * sizeof(double), hipMemcpyDeviceToHost, r->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-  (void)hipFree(d);
-  if (e != hipSuccess) return hip_fail(e, "qs_probabilities");
+  QS_HIP(ensure_big(r, step * sizeof(double), &d));
+  for (uint64_t off = 0; off < r->len; off += step) {
+    hipError_t e = by_dtype(r, [&](auto z) {
+      return qs::launch_probabilities<decltype(z)>((const char *)r->psi + off * r->amp_bytes(),
+                                                   m, d, r->stream);
+    });
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out + off, d, step * sizeof(double), hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return hip_fail(e, "qs_probabilities");
+  }
   return QS_OK;
 }
 

@@ -463,6 +463,28 @@ def test_sample_cdf():
 
 # ---- reductions and equality ---------------------------------------------------------
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_readout_in_pieces(dtype):
+    """At 24 qubits the readouts run through the register's work buffer in
+    pieces of 2^22 amplitudes: set_state / state round trip (complex64
+    narrows and widens per piece), probabilities, and 2^23 + 5 amplitudes by
+    index, against numpy on the same values; a second call reuses the
+    buffer."""
+    n = 24
+    rng = np.random.default_rng(24)
+    psi = random_state(rng, n)
+    r = Register.new([False] * n, dtype)
+    r.state = psi
+    want = psi if dtype == "c128" else psi.astype(np.complex64).astype(np.complex128)
+    for _ in range(2):
+        got = r.state
+        assert np.array_equal(got, want)
+        p = r.probabilities()
+        assert np.max(np.abs(p - np.abs(want) ** 2)) < 1e-15
+        idx = rng.integers(0, 1 << n, (1 << 23) + 5, dtype=np.uint64)
+        assert np.array_equal(r.amplitudes(idx), want[idx])
+
+
 def test_norm_inner_equal_probabilities():
     rng = np.random.default_rng(21)
     n = 15
