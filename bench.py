@@ -298,6 +298,62 @@ def measure_bench(reg: Register, n: int):
             "measure_sequence_all_qubits_ms": t_seq * 1e3}
 
 
+def sweep_ops_bench(n: int, ks=(2, 16), reps: int = 3):
+    """Sweeps of K Haar gates on qubits 3..10 of an n-qubit complex128 state
+    (one sweep each; run with QS_SWEEP_MERGE=0 so that every gate stays one
+    op): the per-sweep time against the op count, for counter runs."""
+    rng = np.random.default_rng(1)
+    r = Register.new([False] * n)
+    res = {}
+    for k in ks:
+        gates = []
+        for i in range(k):
+            z = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+            u, _ = np.linalg.qr(z)
+            gates.append(qops.Operation(u, [3 + i % 8]))
+        dt, ms, cnt, st = timed_run(r, gates, reps, 1)
+        avg = ms[0] / 1e3 / max(cnt[0], 1)
+        res[f"k{k}"] = {"sweeps": st["sweeps"], "ops": st["sweep_ops"], "avg_ms": avg * 1e3,
+                        "frac": 2.0 * (1 << n) * 16 / avg / HBM_PEAK if cnt[0] else None}
+        log(f"sweep_ops k={k}: {res[f'k{k}']}")
+    r.close()
+    return res
+
+
+def block5_bench(n: int, only=None):
+    """Dense complex 32 x 32 blocks (random unitaries) on 5 qubits at n
+    qubits, fusion off: complex64 on the FP32 matrix cores (k_gatek5_mfma),
+    complex128 on the FP64 vector units (k_gatek<double, 5>).  Three blocks
+    each on qubits 10..14 and 20..24; average kernel time and fraction of HBM
+    peak (2 x 2^n x s bytes per launch)."""
+    rng = np.random.default_rng(55)
+    res = {"workload": f"complex 32 x 32 blocks, {n} qubits, qubits 10-14 and 20-24"}
+    for dtype in ("c64", "c128"):
+        if only and dtype != only:
+            continue
+        amp = 16 if dtype == "c128" else 8
+        r = Register.new([False] * n, dtype).set_fusion(False)
+        entry = {}
+        for label, tg in (("q10_14", [10, 11, 12, 13, 14]), ("q20_24", [20, 21, 22, 23, 24])):
+            z = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+            u, _ = np.linalg.qr(z)
+            op = qops.Operation(u, tg)
+            dt, ms, cnt, _ = timed_run(r, [op], 3, 1)
+            avg = ms[3] / 1e3 / max(cnt[3], 1)
+            byt = 2.0 * (1 << n) * amp
+            entry[label] = {"avg_ms": avg * 1e3, "achieved": byt / avg / 1e9, "unit": "GB/s",
+                            "frac": byt / avg / HBM_PEAK,
+                            "tflops": (1 << n) * 32 * 8.0 / avg / 1e12}
+        entry["kernel"] = ("k_gatek5_mfma (v_mfma_f32_32x32x2f32)" if dtype == "c64"
+                           else "k_gatek<double, 5> (FP64 FMA)")
+        res[dtype] = entry
+        log(f"block5 {dtype}: " + ", ".join(f"{k} {v['avg_ms']:.2f} ms"
+                                            for k, v in entry.items() if isinstance(v, dict)))
+        r.close()
+        del r
+    return res
+
+
 def grover_bench(n: int, iterations: int, seed: int, only=None):
     """The Grover workload at n qubits: H on all, then `iterations` x
     [phase flip of the seeded winner, diffusion].  'fused' applies every
@@ -359,12 +415,15 @@ def main() -> int:
                     help="skip the 32-qubit Grover workload (64 GiB complex128, 32 GiB "
                          "complex64): 5-qubit fused blocks vs one gate per qubit")
     ap.add_argument("--no-measure", action="store_true")
+    ap.add_argument("--no-block5", action="store_true",
+                    help="skip the 32-qubit complex 5-qubit block kernels")
     ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
                     help="threads of the multi-core CPU baseline (default: every core this "
                          "process may run on)")
     ap.add_argument("--profile", nargs="?", const="c128", default=None,
                     choices=["c128", "c64", "unfused", "measure", "adjoint", "grover_fused",
-                             "grover_unfused", "grover_c64", "grover_c64_unfused"],
+                             "grover_unfused", "grover_c64", "grover_c64_unfused",
+                             "block5_c64", "block5_c128", "sweep_ops"],
                     help="short run for rocprofv3: a 1 GiB torch copy (counter "
                          "calibration), then one warmup-free step of one section")
     args = ap.parse_args()
@@ -375,12 +434,17 @@ def main() -> int:
         return 2
     sections = {"c128": True, "c64": not args.no_c64, "unfused": not args.no_unfused,
                 "measure": not args.no_measure, "adjoint": not args.no_adjoint,
-                "cpu": not args.no_cpu, "grover32": not args.no_grover32}
+                "cpu": not args.no_cpu, "grover32": not args.no_grover32,
+                "block5": not args.no_block5}
     if args.profile:
         args.steps, args.warmup = 1, 0
         sections = {k: False for k in sections}
         if args.profile.startswith("grover"):
             sections["grover32"] = True
+        elif args.profile.startswith("block5"):
+            sections["block5"] = True
+        elif args.profile == "sweep_ops":
+            pass
         else:
             sections[args.profile] = True
         import torch
@@ -517,6 +581,14 @@ def main() -> int:
             only = {"grover_fused": "c128_fused", "grover_unfused": "c128_unfused",
                     "grover_c64": "c64_fused", "grover_c64_unfused": "c64_unfused"}[args.profile]
         out["grover32"] = grover_bench(32, 12, seed=4, only=only)
+
+    if args.profile == "sweep_ops":
+        out["sweep_ops"] = sweep_ops_bench(n)
+
+    if sections["block5"]:
+        only = args.profile.split("_")[1] if args.profile and args.profile.startswith("block5") \
+            else None
+        out["block5"] = block5_bench(32, only=only)
 
     if sections["cpu"]:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_threads)

@@ -166,8 +166,23 @@ bool fits_set(const Gate &g, uint64_t S, const TileShape &ts, Plan &pl) {
   return true;
 }
 
+// Moves the pattern bits of a controlled op that lie off the tile (`tmask`
+// = the tile qubits) to umask: they are the same for the whole tile, so the
+// kernel decides them once per tile and skips the op where they fail.
+inline void split_tile_controls(OpRec &op, uint64_t tmask) {
+  const uint64_t off = op.cmask & ~tmask;
+  if (off == 0) return;
+  op.umask = off;
+  op.cmask &= tmask;
+  op.code_fail = CODE_NONE;
+  op.uoff = 0;
+}
+
+inline bool is_minus_one(const cd &v) { return v.re == -1.0 && v.im == 0.0; }
+
 template <typename R>
-void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::vector<R> &pool) {
+void encode_gate(const Gate &g, const int *slot, uint64_t tmask, std::vector<OpRec> &ops,
+                 std::vector<R> &pool) {
   auto put = [&](const cd &v) {
     pool.push_back((R)v.re);
     pool.push_back((R)v.im);
@@ -179,8 +194,9 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
       op.cmask = g.smask;
       op.cval = g.spat[e];
       op.moff = (uint32_t)pool.size();
-      op.code = op_code(op.type, 0, 0);
+      op.code = is_minus_one(g.m[e]) ? (uint32_t)CODE_NEG : op_code(op.type, 0, 0);
       put(g.m[e]);
+      split_tile_controls(op, tmask);
       ops.push_back(op);
     }
     return;
@@ -195,7 +211,8 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
     op.na = (uint32_t)g.na;
     for (int j = 0; j < g.na; ++j) op.act[j] = (uint32_t)g.act[j];
     for (const cd &v : g.m) put(v);
-    op.code = op_code(op.type, 0, 0);
+    op.code = (g.na == 0 && is_minus_one(g.m[0])) ? (uint32_t)CODE_NEG : op_code(op.type, 0, 0);
+    split_tile_controls(op, tmask);
     ops.push_back(op);
     return;
   }
@@ -211,13 +228,16 @@ void encode_gate(const Gate &g, const int *slot, std::vector<OpRec> &ops, std::v
   }
   if (op.type != OP_X1)
     for (const cd &v : g.m) put(v);
+  split_tile_controls(op, tmask);
+  // controls all off the tile: where they hold, the plain 2x2
+  if (op.type == OP_C1 && op.cmask == 0) op.type = OP_U1;
   op.code = op_code(op.type, op.s0, op.s1);
   ops.push_back(op);
 }
 
 template <typename R>
-void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, std::vector<OpRec> &ops,
-                std::vector<R> &pool) {
+void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, uint64_t tmask,
+                std::vector<OpRec> &ops, std::vector<R> &pool) {
   OpRec op{};
   op.s0 = (uint32_t)slot[o.t];
   op.moff = (uint32_t)pool.size();
@@ -225,20 +245,38 @@ void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, std::vec
     pool.push_back((R)v.re);
     pool.push_back((R)v.im);
   };
+  const uint64_t pin = o.pm & tmask;  // the pattern's tile bits
   if (o.pm == 0) {
     op.type = OP_U1;
     for (int k = 0; k < 4; ++k) put(o.m1[k]);
     op.code = op_code(op.type, op.s0, 0);
+  } else if (pin == 0) {
+    // the pattern lies off the tile: one plain 2x2 per tile, m1 where the
+    // pattern holds (pool + 8), m0 elsewhere
+    op.type = OP_U1;
+    for (int k = 0; k < 4; ++k) put(o.m0[k]);
+    for (int k = 0; k < 4; ++k) put(o.m1[k]);
+    op.code = op_code(op.type, op.s0, 0);
+    op.cval = o.pv;
+    op.umask = o.pm;
+    op.code_fail = op.code;
+    op.uoff = 8;
   } else {
     op.type = OP_MUX;
-    op.cmask = o.pm;
+    op.cmask = pin;
     op.cval = o.pv;
     for (int k = 0; k < 4; ++k) put(o.m0[k]);
     for (int k = 0; k < 4; ++k) put(o.m1[k]);
-    const int form = mux_form(o.t, o.pm, slot, reg_qubits);
-    if (form == 3) op.s1 = (uint32_t)slot[__builtin_ctzll(o.pm)];
+    const int form = mux_form(o.t, pin, slot, reg_qubits);
+    if (form == 3) op.s1 = (uint32_t)slot[__builtin_ctzll(pin)];
     op.code = op_code(op.type, op.s0, op.s1,
                       form == 1 ? MUX_LANES : (form == 2 ? MUX_REGS : MUX_ONE_REG));
+    if (o.pm != pin) {
+      // where the off-tile bits fail, m0 on the whole tile
+      op.umask = o.pm & ~tmask;
+      op.code_fail = op_code(OP_U1, op.s0, 0);
+      op.uoff = 0;
+    }
   }
   ops.push_back(op);
 }
@@ -540,9 +578,9 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
         continue;
       }
       if (o.g)
-        encode_gate<R>(*o.g, slot, ops, pool);
+        encode_gate<R>(*o.g, slot, P, ops, pool);
       else
-        encode_mux<R>(o, slot, reg_qubits, ops, pool);
+        encode_mux<R>(o, slot, reg_qubits, P, ops, pool);
     }
     h.nops = (uint32_t)ops.size();
     if (run.sweep_ops) *run.sweep_ops += ops.size();

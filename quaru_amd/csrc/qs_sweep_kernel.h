@@ -94,7 +94,10 @@ enum : uint32_t {
   CODE_U1W = CODE_MUXRL + kMaxA * kLaneBits,
   CODE_C1W = CODE_U1W + 1,
   CODE_X1W = CODE_C1W + 1,
-  CODE_MUXW = CODE_X1W + 1  // pattern on no register slot
+  CODE_MUXW = CODE_X1W + 1,  // pattern on no register slot
+  // multiply by -1 where (idx & cmask) == cval (a sparse op whose value is
+  // -1: CZ, CCZ and their kin): sign flips instead of complex products
+  CODE_NEG = CODE_MUXW + 1
 };
 
 // Kernel forms of a multiplexed 2x2 (OP_MUX), by where its pattern lies.
@@ -132,17 +135,23 @@ __host__ __device__ inline uint32_t op_code(uint32_t type, uint32_t s0, uint32_t
   }
 }
 
-// 64-byte op record; complex values live in a pool of R scalars
+// 80-byte op record; complex values live in a pool of R scalars
 // ({re, im} interleaved, row-major) at pool[moff].
 // A batched 2x2 (bmask != 0) takes its matrix at pool[moff + 8 b] for the
 // tile's batch index b = (tile base >> bshift) & bmask; the batch bits are
 // never tile qubits, so b is uniform over the tile.
+// Pattern bits off the tile (umask) are the same for every amplitude of a
+// tile: where (tile base & umask) == (cval & umask) the op runs as `code`
+// with its values at pool[moff + uoff], elsewhere as `code_fail` (CODE_NONE:
+// skipped) at pool[moff].  cmask then holds the pattern's tile bits only.
 struct OpRec {
   uint32_t type, s0, s1, na;
   uint32_t act[4];
   uint32_t moff, code;
   uint64_t cmask, cval;
   uint32_t bshift, bmask;
+  uint64_t umask;
+  uint32_t code_fail, uoff;
 };
 
 struct SweepHeader {
@@ -881,6 +890,25 @@ __device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const QS_CONST OpRe
   }
 }
 
+// Sign flip where the pattern holds: the high word of each value carries
+// the sign, so one select per amplitude and two integer xors.
+template <int A, typename R, typename I>
+__device__ __forceinline__ void op_neg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                       const Ctx<I> &c, const Regs<A, I> &g) {
+  const Pred<I> p = make_pred<A>(op, c, g);
+#pragma unroll
+  for (int r = 0; r < (1 << A); ++r) {
+    const uint32_t flip = pred_on<A>(p, g, r) ? 0x80000000u : 0u;
+    if constexpr (sizeof(R) == 8) {
+      a.re[r] = __hiloint2double(__double2hiint(a.re[r]) ^ (int)flip, __double2loint(a.re[r]));
+      a.im[r] = __hiloint2double(__double2hiint(a.im[r]) ^ (int)flip, __double2loint(a.im[r]));
+    } else {
+      a.re[r] = __int_as_float(__float_as_int(a.re[r]) ^ (int)flip);
+      a.im[r] = __int_as_float(__float_as_int(a.im[r]) ^ (int)flip);
+    }
+  }
+}
+
 template <int A, typename R, typename I>
 __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                         const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
@@ -945,10 +973,21 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
   for (int o = 0; o < nops; ++o) {
     const QS_CONST OpRec &op = ops[o];
     const QS_CONST R *m = pool + op.moff;
-    if (op.bmask)  // the tile base is the same for every lane of a wave
+    uint32_t code = op.code;
+    // the tile base is the same for every lane of a wave: read it from the
+    // first lane, so that the choices below stay scalar
+    if (op.umask) {
+      const uint32_t tok = __builtin_amdgcn_readfirstlane(
+          (uint32_t)((((uint64_t)c.tbase ^ op.cval) & op.umask) == 0));
+      if (tok)
+        m += op.uoff;
+      else
+        code = op.code_fail;
+    }
+    if (op.bmask)
       m += 8 * __builtin_amdgcn_readfirstlane(
                    (uint32_t)(((uint64_t)c.tbase >> op.bshift) & op.bmask));
-    switch (op.code) {
+    switch (code) {
       QS_REG1(CODE_U1, 0, (op_1q_reg<0, false, A>(a, op, m, c, g)))
       QS_REG1(CODE_U1, 1, (op_1q_reg<1, false, A>(a, op, m, c, g)))
       QS_REG1(CODE_U1, 2, (op_1q_reg<2, false, A>(a, op, m, c, g)))
@@ -968,6 +1007,7 @@ __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       QS_REG1(CODE_X1, 4, (op_x1_reg<4, A, R, I>(a, op, c, g)))
       QS_LANE6(CODE_X1 + kMaxA, (op_x1_lane<LK, A, R, I>(a, op, c, g)))
       case CODE_SPARSE: op_sparse<A>(a, op, m, c, g); break;
+      case CODE_NEG: op_neg<A, R, I>(a, op, c, g); break;
       case CODE_DIAG: op_diag<A>(a, op, m, c, g); break;
       QS_REG2_ROW(0)
       QS_REG2_ROW(1)

@@ -132,6 +132,9 @@ inline bool pattern_2x2(const Gate &g, int choice, int &t, uint64_t &pm, uint64_
 // register slot; 3 = on one register slot alone, t on a lane bit; 0 = none.
 inline int mux_form(int t, uint64_t pm, const int *slot, uint64_t reg_qubits) {
   if (t < 0 || slot[t] < 0) return 0;
+  // bits off the tile are decided once per tile (the op's umask)
+  for (uint64_t b = pm; b; b &= b - 1)
+    if (slot[__builtin_ctzll(b)] < 0) pm &= ~(b & (~b + 1));
   if ((pm & reg_qubits) == 0) return 1;
   if (pm & ~reg_qubits) return 0;
   if (slot[t] < kFirstLaneSlot) return 2;

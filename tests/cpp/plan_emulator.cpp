@@ -138,9 +138,35 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
   auto val = [&](uint32_t off, int k) {
     return C((double)pool[off + 2 * k], (double)pool[off + 2 * k + 1]);
   };
-  for (const OpRec &o : ops) {
+  // an op with pattern bits off the tile runs as `code` (values at moff +
+  // uoff) on the tiles where they hold and as `code_fail` (values at moff)
+  // on the others
+  uint64_t sel_mask = 0, sel_val = 0;
+  bool sel_want = true;
+  auto sel = [&](uint64_t i) { return (((i ^ sel_val) & sel_mask) == 0) == sel_want; };
+  for (const OpRec &o0 : ops) {
+    if (g_hist && o0.code < 128) ++g_hist[o0.code];
+    if (o0.umask & tmask) return fail("uniform pattern bits on the tile");
+    if (o0.umask == 0 && (o0.code_fail != 0 || o0.uoff != 0)) return fail("stray fail code");
+    for (int pass = 0; pass < (o0.umask ? 2 : 1); ++pass) {
+    OpRec o = o0;
+    sel_mask = o0.umask;
+    sel_val = o0.cval;
+    sel_want = pass == 0;
+    if (o0.umask) {
+      if (pass == 0)
+        o.moff += o0.uoff;
+      else
+        o.code = o0.code_fail;
+    }
     const uint32_t code = o.code;
-    if (g_hist && code < 128) ++g_hist[code];
+    if (code == CODE_NONE) continue;
+    if (code == CODE_NEG) {
+      if (!plan_only && (size_t)o.moff + 2 > pool.size()) return fail("pool bounds");
+      for (uint64_t i = 0; i < N; ++i)
+        if (sel(i) && (i & o.cmask) == (o.cval & o.cmask)) psi[i] = -psi[i];
+      continue;
+    }
     int kind = 0, j = 0;
     int lane_code = -1;  // lane bit a lane-target code names
     if (code >= CODE_U1 && code < CODE_U1 + kLaneCodes) kind = OP_U1, j = code - CODE_U1;
@@ -193,7 +219,7 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
         m1[k] = val(o.moff, 4 + k);
       }
       for (uint64_t i = 0; i < N; ++i) {
-        if (i & bt) continue;
+        if ((i & bt) || !sel(i)) continue;
         const bool on = (i & bj) == (o.cval & bj);
         const C *m = on ? m1 : m0;
         const C x = psi[i], y = psi[i | bt];
@@ -213,7 +239,7 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
         if (!need_pool(8 * (o.bmask + 1))) return false;
         if (tmask & ((uint64_t)o.bmask << o.bshift)) return fail("batch bit on the tile");
         for (uint64_t i = 0; i < N; ++i) {
-          if ((i & bt) || (kind == OP_C1 && (i & cm) != cv)) continue;
+          if ((i & bt) || !sel(i) || (kind == OP_C1 && (i & cm) != cv)) continue;
           const uint32_t mo = o.moff + 8 * (uint32_t)((i >> o.bshift) & o.bmask);
           const C x = psi[i], y = psi[i | bt];
           psi[i] = val(mo, 0) * x + val(mo, 1) * y;
@@ -235,7 +261,7 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
         }
       }
       for (uint64_t i = 0; i < N; ++i) {
-        if (i & bt) continue;
+        if ((i & bt) || !sel(i)) continue;
         bool on;
         if (kind == OP_U1) on = true;
         else if (kind == 100) on = (i & cm & ~regmask) == (cv & ~regmask);  // lane_ok only
@@ -254,7 +280,7 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
       if (!need_pool(32)) return false;
       const uint64_t b0 = h.gb[s0], b1 = h.gb[s1];
       for (uint64_t i = 0; i < N; ++i) {
-        if ((i & b0) || (i & b1) || (i & cm) != cv) continue;
+        if ((i & b0) || (i & b1) || !sel(i) || (i & cm) != cv) continue;
         const uint64_t id[4] = {i, i | b0, i | b1, i | b0 | b1};
         C x[4], y[4];
         for (int k = 0; k < 4; ++k) x[k] = psi[id[k]];
@@ -268,16 +294,17 @@ bool emulate(std::vector<C> &psi, int n, int A, const void *blob, size_t bytes) 
       if (!need_pool(2)) return false;
       const C v = val(o.moff, 0);
       for (uint64_t i = 0; i < N; ++i)
-        if ((i & cm) == cv) psi[i] *= v;
+        if (sel(i) && (i & cm) == cv) psi[i] *= v;
     } else if (kind == OP_DIAG) {
       if (o.na < 1 || o.na > 2) return fail("diag arity");
       if (!need_pool(2u << o.na)) return false;
       const int q0 = (int)o.act[0], q1 = o.na > 1 ? (int)o.act[1] : (int)o.act[0];
       for (uint64_t i = 0; i < N; ++i) {
-        if ((i & cm) != cv) continue;
+        if (!sel(i) || (i & cm) != cv) continue;
         const int b0 = (i >> q0) & 1, b1 = o.na > 1 ? (int)((i >> q1) & 1) : 0;
         psi[i] *= val(o.moff, b0 + 2 * b1);
       }
+    }
     }
   }
   return true;
