@@ -217,15 +217,12 @@ def adjoint_bench(n: int, layers: int, batch: int, fd_params: int, dtype: str):
     t2 = time.perf_counter()
     t_adj, t_fwd, t_bwd = t2 - t0, t1 - t0, t2 - t1
     # algorithmic HBM bytes of the reverse pass (S = the batch of states):
-    # lambda = h psi (2S), per rotation the fused reverse step reads and
-    # writes psi and lambda (4S), per CZ the inverse phase touches a quarter
-    # of psi and of lambda (S), per checkpoint restore a copy (2S)
+    # lambda = h psi (2S), each tiled block reads and writes psi and lambda
+    # (4S), each checkpoint restore copies psi (2S)
     bp = 1 << max(0, (batch - 1).bit_length())
     S = float(bp << n) * (16 if dtype == "c128" else 8)
-    nrot = sum(1 for op in ans.ops if op[0] in ("rx", "ry", "rz"))
-    ncz = sum(1 for op in ans.ops if op[0] == "cz")
-    restores = max(0, (layers - 1) // ckpt)
-    bwd_bytes = 2 * S + nrot * 4 * S + ncz * S + restores * 2 * S
+    rstats = run.reverse_stats
+    bwd_bytes = 2 * S + rstats["blocks"] * 4 * S + rstats["restores"] * 2 * S
     fwd_bytes = fstats["sweeps"] * 2 * S
     # central finite differences (h = 1e-4, FP64) on the same GPU for a sample
     # of parameters: each needs two forward passes of the whole batch.
@@ -253,8 +250,9 @@ def adjoint_bench(n: int, layers: int, batch: int, fd_params: int, dtype: str):
         "backward_seconds": t_bwd,
         "forward_sweeps": fstats["sweeps"],
         "forward_kernels": fstats["kernels"],
+        "reverse_blocks": rstats["blocks"],
         "roofline": {
-            "bound": "hbm", "kernel": "k_rot_reverse (fused reverse step on psi and lambda), "
+            "bound": "hbm", "kernel": "k_reverse_tile (tiled reverse blocks on psi and lambda), "
                                       "the reverse pass as a whole",
             "achieved": bwd_bytes / t_bwd / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": bwd_bytes / t_bwd / HBM_PEAK, "bytes": bwd_bytes,

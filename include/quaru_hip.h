@@ -278,6 +278,10 @@ int qs_adjoint_backward(qs_adjoint *ctx, double *grads);
  * reference counterpart; diagnostics). */
 int qs_adjoint_stats(const qs_adjoint *ctx, uint64_t *sweeps, uint64_t *kernels);
 
+/* Tiled passes over psi and lambda, and checkpoint restores, of the last
+ * backward of *ctx (no reference counterpart; diagnostics). */
+int qs_adjoint_reverse_stats(const qs_adjoint *ctx, uint64_t *blocks, uint64_t *restores);
+
 void qs_adjoint_free(qs_adjoint *ctx);
 
 #ifdef __cplusplus

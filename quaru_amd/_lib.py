@@ -81,6 +81,7 @@ SIGNATURES = {
     "qs_adjoint_forward": (_I, [_P, _P, _S, _P, _P, _S, _I, _S, _PP, _P]),
     "qs_adjoint_backward": (_I, [_P, _P]),
     "qs_adjoint_stats": (_I, [_P, _P, _P]),
+    "qs_adjoint_reverse_stats": (_I, [_P, _P, _P]),
     "qs_adjoint_free": (None, [_P]),
 }
 

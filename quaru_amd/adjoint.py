@@ -95,6 +95,7 @@ class AdjointRun:
         self._ctx = ctx
         self.batch = batch
         self.nparams = nparams
+        self.reverse_stats = None  # set by backward()
 
     def close(self):
         if self._ctx is not None and self._ctx.value:
@@ -121,7 +122,12 @@ class AdjointRun:
             raise RuntimeError("backward() already ran for this forward pass")
         g = np.zeros((self.batch, self.nparams), dtype=np.float64)
         try:
-            _check(_lib.load().qs_adjoint_backward(self._ctx, _vp(g)))
+            lib = _lib.load()
+            _check(lib.qs_adjoint_backward(self._ctx, _vp(g)))
+            bl, rs = ctypes.c_uint64(0), ctypes.c_uint64(0)
+            _check(lib.qs_adjoint_reverse_stats(self._ctx, ctypes.byref(bl), ctypes.byref(rs)))
+            # tiled passes over psi and lambda, checkpoint restores
+            self.reverse_stats = {"blocks": bl.value, "restores": rs.value}
         finally:
             self.close()
         return g

@@ -259,6 +259,245 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(double *p, uint64_t count)
     p[i] = 0.0;
 }
 
+// ---- fused reverse pass.  The reverse sweep runs in blocks of consecutive
+// operations whose pair targets fit one tile of T qubits (the low qubits
+// always among them, for contiguous loads).  A workgroup loads its tile of
+// psi and of lambda into LDS, runs the block's reverse steps there (each
+// rotation: the tile's share of Im <lambda|sigma|psi>, then U^dagger on
+// both; each fixed gate: its inverse on both) and writes the tile back:
+// one pass over psi and lambda per block instead of one per operation.
+// Diagonal and phase gates act on any qubits (their bits off the tile are
+// the same for the whole tile).
+enum : uint32_t { REV_ROT = 1, REV_PHASE = 2, REV_DIAG = 3, REV_U1 = 4 };
+
+struct RevRec {
+  uint32_t kind;
+  uint32_t k;      // tile bit of the target (REV_ROT, REV_U1)
+  uint32_t axis;   // REV_ROT: rotation axis; REV_DIAG: number of act bits
+  uint32_t rslot;  // REV_ROT: rotation slot of the matrix table
+  uint32_t out;    // REV_ROT: row of the partial sums
+  uint32_t moff;   // values at vals[moff..] as {re, im} pairs
+  int32_t a0, a1;  // REV_DIAG: act bits: tile bit (>= 0) or -1 - global qubit
+  uint64_t umask, uval;  // pattern bits off the tile (global index bits)
+  uint64_t lmask, lval;  // pattern bits on the tile (tile bits)
+};
+
+constexpr int kRevMaxOps = 64;
+constexpr int kRevMaxTile = 16;
+constexpr int kRevMaxBits = 11;  // tile of at most 2^11 amplitudes
+
+struct RevHeader {
+  uint32_t T, nops, nrot, nvals;
+  uint32_t tq[kRevMaxTile];  // tile qubits, ascending
+};
+
+// One workgroup per tile (grid = ntiles); dynamic LDS 2 x 2^T amplitudes;
+// PER = max(1, 2^T / 256).  part[out * ntiles + tile] = the tile's share of
+// rotation `out`'s gradient.
+template <typename R, int PER>
+__global__ __launch_bounds__(kBlock) void k_reverse_tile(typename AV2<R>::T *__restrict__ psi,
+                                                         typename AV2<R>::T *__restrict__ lam,
+                                                         int n, const char *__restrict__ blob,
+                                                         const typename AV2<R>::T *__restrict__ rot,
+                                                         uint64_t Bp, double *__restrict__ part,
+                                                         uint64_t ntiles) {
+  using T = typename AV2<R>::T;
+  extern __shared__ unsigned char rev_smem[];
+  __shared__ double red[2][kBlock / 64];
+  const RevHeader &h = *(const RevHeader *)blob;
+  const RevRec *recs = (const RevRec *)(blob + sizeof(RevHeader));
+  const R *vals = (const R *)(blob + sizeof(RevHeader) + h.nops * sizeof(RevRec));
+  const int Tt = (int)h.T;
+  const uint32_t E = 1u << Tt;
+  T *ps = (T *)rev_smem;
+  T *ls = ps + E;
+  const uint64_t tile = blockIdx.x;
+  uint64_t base = tile;
+  for (int i = 0; i < Tt; ++i) base = insert0(base, (int)h.tq[i]);
+  const uint64_t b = base >> n;
+  // Amplitude j = threadIdx.x + 256 i of the tile: the thread index gives
+  // its low 8 tile bits, i the rest (the same for the whole workgroup).  All
+  // loads of the tile are issued before the first is waited for.
+  const int tb = Tt < 8 ? Tt : 8;
+  uint64_t toff = 0;
+  for (int i = 0; i < tb; ++i)
+    if ((threadIdx.x >> i) & 1u) toff |= 1ull << h.tq[i];
+  // PER amplitudes per thread (2^T / 256, at least 1: a tile of fewer than
+  // 256 amplitudes leaves the other threads idle)
+  const bool active = threadIdx.x < E;
+  uint64_t ioff[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    uint64_t o = 0;
+    for (int k = 8; k < Tt; ++k)
+      if ((i >> (k - 8)) & 1) o |= 1ull << h.tq[k];
+    ioff[i] = base | toff | o;
+  }
+  typedef R V __attribute__((ext_vector_type(2)));
+  const V *psv = reinterpret_cast<const V *>(psi), *lav = reinterpret_cast<const V *>(lam);
+  V *psl = reinterpret_cast<V *>(ps), *lsl = reinterpret_cast<V *>(ls);
+  if (active) {
+    V xv[PER], lv[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      xv[i] = psv[ioff[i]];
+      lv[i] = lav[ioff[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      psl[threadIdx.x + i * kBlock] = xv[i];
+      lsl[threadIdx.x + i * kBlock] = lv[i];
+    }
+  }
+  __syncthreads();
+  for (uint32_t o = 0; o < h.nops; ++o) {
+    const RevRec &r = recs[o];
+    if (r.kind == REV_ROT) {
+      const T *mm = rot + ((uint64_t)r.rslot * Bp + b) * 4;
+      const T m0 = mm[0], m1 = mm[1], m2 = mm[2], m3 = mm[3];
+      const uint32_t bk = 1u << r.k;
+      double acc = 0.0;
+#pragma unroll
+      for (int pi = 0; pi < (PER + 1) / 2; ++pi) {
+        const uint32_t p = threadIdx.x + (uint32_t)pi * kBlock;
+        if (p >= E / 2) break;
+        const uint32_t j0 = (uint32_t)insert0(p, (int)r.k), j1 = j0 | bk;
+        const T x0 = ps[j0], x1 = ps[j1], l0 = ls[j0], l1 = ls[j1];
+        double s0r, s0i, s1r, s1i;  // sigma . psi
+        if (r.axis == 0) {
+          s0r = x1.x; s0i = x1.y; s1r = x0.x; s1i = x0.y;
+        } else if (r.axis == 1) {
+          s0r = x1.y; s0i = -(double)x1.x; s1r = -(double)x0.y; s1i = x0.x;
+        } else {
+          s0r = x0.x; s0i = x0.y; s1r = -(double)x1.x; s1i = -(double)x1.y;
+        }
+        acc += (double)l0.x * s0i - (double)l0.y * s0r + (double)l1.x * s1i - (double)l1.y * s1r;
+        // U^dagger = [[conj m0, conj m2], [conj m1, conj m3]]
+        T y0, y1, k0, k1;
+        y0.x = m0.x * x0.x + m0.y * x0.y + m2.x * x1.x + m2.y * x1.y;
+        y0.y = m0.x * x0.y - m0.y * x0.x + m2.x * x1.y - m2.y * x1.x;
+        y1.x = m1.x * x0.x + m1.y * x0.y + m3.x * x1.x + m3.y * x1.y;
+        y1.y = m1.x * x0.y - m1.y * x0.x + m3.x * x1.y - m3.y * x1.x;
+        k0.x = m0.x * l0.x + m0.y * l0.y + m2.x * l1.x + m2.y * l1.y;
+        k0.y = m0.x * l0.y - m0.y * l0.x + m2.x * l1.y - m2.y * l1.x;
+        k1.x = m1.x * l0.x + m1.y * l0.y + m3.x * l1.x + m3.y * l1.y;
+        k1.y = m1.x * l0.y - m1.y * l0.x + m3.x * l1.y - m3.y * l1.x;
+        ps[j0] = y0;
+        ps[j1] = y1;
+        ls[j0] = k0;
+        ls[j1] = k1;
+      }
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+      // two buffers by row parity: a buffer is rewritten two rotations
+      // later, after the barrier of the rotation between
+      double *rb = red[r.out & 1u];
+      if ((threadIdx.x & 63u) == 0) rb[threadIdx.x >> 6] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += rb[w];
+        part[(uint64_t)r.out * ntiles + tile] = t;
+      }
+    } else if (r.kind == REV_PHASE || r.kind == REV_DIAG) {
+      // a run of consecutive phase and diagonal records: one LDS pass, each
+      // amplitude multiplied by every factor whose pattern it meets
+      uint32_t e = o + 1;
+      while (e < h.nops && (recs[e].kind == REV_PHASE || recs[e].kind == REV_DIAG)) ++e;
+#pragma unroll
+      for (int ji = 0; ji < PER; ++ji) {
+        const uint32_t j = threadIdx.x + (uint32_t)ji * kBlock;
+        if (j >= E) break;
+        T x = ps[j], l = ls[j];
+        for (uint32_t t = o; t < e; ++t) {
+          const RevRec &d = recs[t];
+          if ((base & d.umask) != d.uval || (j & (uint32_t)d.lmask) != (uint32_t)d.lval) continue;
+          uint32_t vo = d.moff;
+          if (d.kind == REV_DIAG) {
+            const uint32_t i0 =
+                d.a0 >= 0 ? (j >> d.a0) & 1u : (uint32_t)((base >> (-1 - d.a0)) & 1ull);
+            const uint32_t i1 =
+                d.axis < 2 ? 0u
+                           : (d.a1 >= 0 ? (j >> d.a1) & 1u : (uint32_t)((base >> (-1 - d.a1)) & 1ull));
+            vo += 2u * (i0 + 2u * i1);
+          }
+          const R vr = vals[vo], vi = vals[vo + 1];
+          T y, k;
+          y.x = vr * x.x - vi * x.y;
+          y.y = vr * x.y + vi * x.x;
+          k.x = vr * l.x - vi * l.y;
+          k.y = vr * l.y + vi * l.x;
+          x = y;
+          l = k;
+        }
+        ps[j] = x;
+        ls[j] = l;
+      }
+      __syncthreads();
+      o = e - 1;
+    } else if (r.kind == REV_U1) {
+      if ((base & r.umask) != r.uval) continue;  // the same for the whole workgroup
+      const R *v = vals + r.moff;  // row-major 2x2 of the inverse gate
+      const uint32_t bk = 1u << r.k;
+#pragma unroll
+      for (int pi = 0; pi < (PER + 1) / 2; ++pi) {
+        const uint32_t p = threadIdx.x + (uint32_t)pi * kBlock;
+        if (p >= E / 2) break;
+        const uint32_t j0 = (uint32_t)insert0(p, (int)r.k), j1 = j0 | bk;
+        if ((j0 & (uint32_t)r.lmask) != (uint32_t)r.lval) continue;
+        T z[2][2] = {{ps[j0], ps[j1]}, {ls[j0], ls[j1]}};
+        for (int s = 0; s < 2; ++s) {
+          const T x0 = z[s][0], x1 = z[s][1];
+          T y0, y1;
+          y0.x = v[0] * x0.x - v[1] * x0.y + v[2] * x1.x - v[3] * x1.y;
+          y0.y = v[0] * x0.y + v[1] * x0.x + v[2] * x1.y + v[3] * x1.x;
+          y1.x = v[4] * x0.x - v[5] * x0.y + v[6] * x1.x - v[7] * x1.y;
+          y1.y = v[4] * x0.y + v[5] * x0.x + v[6] * x1.y + v[7] * x1.x;
+          z[s][0] = y0;
+          z[s][1] = y1;
+        }
+        ps[j0] = z[0][0];
+        ps[j1] = z[0][1];
+        ls[j0] = z[1][0];
+        ls[j1] = z[1][1];
+      }
+      __syncthreads();
+    }
+  }
+  if (active) {
+    V *pso = reinterpret_cast<V *>(psi), *lao = reinterpret_cast<V *>(lam);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      pso[ioff[i]] = psl[threadIdx.x + i * kBlock];
+      lao[ioff[i]] = lsl[threadIdx.x + i * kBlock];
+    }
+  }
+}
+
+// grad[b * P + params[row]] += sum of the tiles of state b in row `row` of
+// part (rows in order, so a parameter shared by two rotations of the block
+// is summed without a race).  grid = Bp.
+__global__ __launch_bounds__(kBlock) void k_rev_accum(const double *__restrict__ part,
+                                                      uint32_t nrot, uint64_t ntiles,
+                                                      uint64_t tiles_per_state,
+                                                      const uint32_t *__restrict__ params,
+                                                      double *__restrict__ grad, size_t P) {
+  __shared__ double sh[kBlock];
+  const uint64_t b = blockIdx.x;
+  for (uint32_t row = 0; row < nrot; ++row) {
+    const double *pr = part + (uint64_t)row * ntiles + b * tiles_per_state;
+    double v = 0.0;
+    for (uint64_t i = threadIdx.x; i < tiles_per_state; i += kBlock) v += pr[i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[b * P + params[row]] += sh[0];
+    __syncthreads();
+  }
+}
+
 unsigned grid_of(uint64_t work, uint64_t per, unsigned cap = 262144) {
   uint64_t g = (work + per - 1) / per;
   if (g < 1) g = 1;
@@ -293,6 +532,8 @@ struct qs_adjoint {
   double *d_part = nullptr;
   size_t part_count = 0;
   double *d_grad = nullptr;
+  double *d_rpart = nullptr;  // partial sums of the tiled reverse blocks
+  uint64_t rev_blocks = 0, rev_restores = 0;  // of the last backward
   // descriptor ring of the fused forward pass (device + pinned host)
   char *d_ring = nullptr, *h_ring = nullptr;
   size_t ring_cap = 0, ring_off = 0;
@@ -315,6 +556,7 @@ void free_ctx(qs_adjoint *c) {
   if (c->d_gmat) (void)hipFree(c->d_gmat);
   if (c->d_part) (void)hipFree(c->d_part);
   if (c->d_grad) (void)hipFree(c->d_grad);
+  if (c->d_rpart) (void)hipFree(c->d_rpart);
   if (c->d_ring) (void)hipFree(c->d_ring);
   if (c->h_ring) (void)hipHostFree(c->h_ring);
   if (c->engine) qs::sweep_engine_free(c->engine);
@@ -618,6 +860,225 @@ int forward_t(qs_adjoint *c, const double *thetas, const uint64_t *masks, const 
   return QS_OK;
 }
 
+// Whether the reverse pass runs in tiled blocks (QS_ADJOINT_FUSED_REVERSE,
+// default on) or one kernel per operation.
+bool adjoint_fused_reverse() {
+  static const bool on = [] {
+    const char *v = std::getenv("QS_ADJOINT_FUSED_REVERSE");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
+// Tile qubits of the reverse blocks: QS_ADJOINT_TILE, default 10 for
+// complex128 and 11 for complex64 (32 KiB of LDS for psi and lambda).
+int reverse_tile_bits(int dtype) {
+  const char *v = std::getenv("QS_ADJOINT_TILE");
+  const int t = v ? std::atoi(v) : (dtype == QS_C128 ? 10 : 11);
+  return std::max(1, std::min(t, kRevMaxBits));
+}
+
+// The reverse sweep in tiled blocks (k_reverse_tile): operations are taken
+// in reverse order while their pair targets fit the tile; a checkpoint
+// restore, a gate with no tile form (two or more dense target qubits) or a
+// full record list ends a block.
+template <typename R>
+int reverse_blocks(qs_adjoint *c) {
+  using T = typename AV2<R>::T;
+  hipError_t e = hipSuccess;
+  const int n = c->n;
+  const size_t Bp = c->Bp, P = c->P;
+  const int Tt = std::min(reverse_tile_bits(c->dtype), n);
+  const int low = std::min(Tt, c->dtype == QS_C128 ? 4 : 5);  // 256-byte runs
+  const uint64_t tiles_per_state = 1ull << (n - Tt);
+  const uint64_t ntiles = tiles_per_state * Bp;
+  // rows of partial sums per block: at most 32, within 64 MiB
+  const uint64_t rmax = std::max<uint64_t>(1, std::min<uint64_t>(32, (64ull << 20) / (8 * ntiles)));
+  if (!c->d_rpart) AJ(hipMalloc((void **)&c->d_rpart, rmax * ntiles * sizeof(double)));
+  if (!c->d_ring) {
+    c->ring_cap = 8u << 20;
+    AJ(hipMalloc((void **)&c->d_ring, c->ring_cap));
+    AJ(hipHostMalloc((void **)&c->h_ring, c->ring_cap));
+  }
+  const uint64_t lowset = (1ull << low) - 1ull;
+  // the block under construction; qubit fields hold global qubits until
+  // the tile is fixed
+  struct Pend {
+    RevRec r;
+    uint64_t cmask, cval;  // global pattern
+    int q0, q1;            // global target / act qubits
+  };
+  std::vector<Pend> pend;
+  std::vector<R> vals;
+  std::vector<uint32_t> params;
+  uint64_t S = lowset;  // tile qubits taken
+  auto put = [&](const qs::cd &v) {
+    vals.push_back((R)v.re);
+    vals.push_back((R)v.im);
+  };
+  auto flush = [&]() -> int {
+    if (pend.empty()) return QS_OK;
+    RevHeader hd{};
+    hd.T = (uint32_t)Tt;
+    uint64_t full = S;
+    for (int q = 0; q < n && __builtin_popcountll(full) < Tt; ++q) full |= 1ull << q;
+    int loc[64];
+    int t = 0;
+    for (int q = 0; q < 64; ++q) {
+      loc[q] = -1;
+      if ((full >> q) & 1ull) {
+        hd.tq[t] = (uint32_t)q;
+        loc[q] = t++;
+      }
+    }
+    std::vector<RevRec> recs;
+    for (const Pend &pd : pend) {
+      RevRec r = pd.r;
+      const uint64_t off = pd.cmask & ~full;
+      r.umask = off;
+      r.uval = pd.cval & off;
+      r.lmask = r.lval = 0;
+      for (uint64_t bits = pd.cmask & full; bits; bits &= bits - 1) {
+        const int q = __builtin_ctzll(bits);
+        r.lmask |= 1ull << loc[q];
+        if ((pd.cval >> q) & 1ull) r.lval |= 1ull << loc[q];
+      }
+      if (r.kind == REV_ROT || r.kind == REV_U1) r.k = (uint32_t)loc[pd.q0];
+      if (r.kind == REV_DIAG) {
+        r.a0 = loc[pd.q0] >= 0 ? loc[pd.q0] : -1 - pd.q0;
+        r.a1 = pd.q1 < 0 ? 0 : (loc[pd.q1] >= 0 ? loc[pd.q1] : -1 - pd.q1);
+      }
+      recs.push_back(r);
+    }
+    hd.nops = (uint32_t)recs.size();
+    hd.nrot = (uint32_t)params.size();
+    hd.nvals = (uint32_t)vals.size();
+    std::vector<char> blob(sizeof(RevHeader) + recs.size() * sizeof(RevRec) +
+                           vals.size() * sizeof(R) + params.size() * sizeof(uint32_t) + 16);
+    char *w = blob.data();
+    std::memcpy(w, &hd, sizeof hd);
+    w += sizeof hd;
+    std::memcpy(w, recs.data(), recs.size() * sizeof(RevRec));
+    w += recs.size() * sizeof(RevRec);
+    if (!vals.empty()) std::memcpy(w, vals.data(), vals.size() * sizeof(R));
+    w += vals.size() * sizeof(R);
+    const size_t poff = (size_t)(w - blob.data() + 3) & ~(size_t)3;
+    if (!params.empty()) std::memcpy(blob.data() + poff, params.data(), params.size() * 4);
+    void *d_blob = nullptr;
+    AJ(ring_upload(c, blob.data(), blob.size(), &d_blob));
+    const size_t lds = 2 * ((size_t)1 << Tt) * sizeof(T);
+    auto kern = Tt <= 8 ? k_reverse_tile<R, 1>
+                        : (Tt == 9 ? k_reverse_tile<R, 2>
+                                   : (Tt == 10 ? k_reverse_tile<R, 4> : k_reverse_tile<R, 8>));
+    hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(kBlock), lds, c->stream, (T *)c->psi,
+                       (T *)c->lam, n, (const char *)d_blob, (const T *)c->d_rot, (uint64_t)Bp,
+                       c->d_rpart, ntiles);
+    AJ(hipGetLastError());
+    ++c->rev_blocks;
+    if (!params.empty()) {
+      hipLaunchKernelGGL(k_rev_accum, dim3((unsigned)Bp), dim3(kBlock), 0, c->stream,
+                         (const double *)c->d_rpart, (uint32_t)params.size(), ntiles,
+                         tiles_per_state, (const uint32_t *)((char *)d_blob + poff), c->d_grad,
+                         P);
+      AJ(hipGetLastError());
+    }
+    pend.clear();
+    vals.clear();
+    params.clear();
+    S = lowset;
+    return QS_OK;
+  };
+  // pair targets must be tile qubits
+  auto fits = [&](int q) { return ((S >> q) & 1ull) || __builtin_popcountll(S) < Tt; };
+  int ck = (int)c->ckpt.size() - 1;
+  for (size_t ii = c->circ.ops.size(); ii-- > 0;) {
+    if (ck >= 0 && c->ckpt_at[ck] == ii + 1) {
+      int rc = flush();
+      if (rc != QS_OK) return rc;
+      AJ(hipMemcpyAsync(c->psi, c->ckpt[ck], c->state_bytes(), hipMemcpyDeviceToDevice,
+                        c->stream));
+      ++c->rev_restores;
+      --ck;
+    }
+    const POp &op = c->circ.ops[ii];
+    const qs::Gate &g = op.inv;
+    int need = 1;  // records this operation adds
+    bool tile_form = true;
+    int pq = -1;  // its pair target
+    if (op.kind == 1) {
+      pq = op.target;
+    } else if (g.kind == qs::Gate::kIdentity) {
+      continue;
+    } else if (g.kind == qs::Gate::kSparse) {
+      need = (int)g.spat.size();
+    } else if (g.kind == qs::Gate::kDiag) {
+      tile_form = g.na >= 1 && g.na <= 2;
+    } else if (g.kind == qs::Gate::kDense && g.na == 1) {
+      pq = g.act[0];
+    } else {
+      tile_form = false;
+    }
+    if (!tile_form) {
+      int rc = flush();
+      if (rc != QS_OK) return rc;
+      const void *tab = (const char *)c->d_gmat + c->gmat_off[2 * ii + 1];
+      AJ(apply_fixed<R>(c, c->psi, g, tab));
+      AJ(apply_fixed<R>(c, c->lam, g, tab));
+      continue;
+    }
+    if ((int)pend.size() + need > kRevMaxOps || (pq >= 0 && !fits(pq)) ||
+        (op.kind == 1 && params.size() >= rmax)) {
+      int rc = flush();
+      if (rc != QS_OK) return rc;
+    }
+    if (pq >= 0) S |= 1ull << pq;
+    if (op.kind == 1) {
+      Pend pd{};
+      pd.r.kind = REV_ROT;
+      pd.r.axis = (uint32_t)op.axis;
+      pd.r.rslot = (uint32_t)op.rot_slot;
+      pd.r.out = (uint32_t)params.size();
+      pd.q0 = op.target;
+      pd.q1 = -1;
+      params.push_back((uint32_t)op.param);
+      pend.push_back(pd);
+    } else if (g.kind == qs::Gate::kSparse) {
+      for (size_t k = 0; k < g.spat.size(); ++k) {
+        Pend pd{};
+        pd.r.kind = REV_PHASE;
+        pd.r.moff = (uint32_t)vals.size();
+        put(g.m[k]);
+        pd.cmask = g.smask;
+        pd.cval = g.spat[k];
+        pd.q0 = pd.q1 = -1;
+        pend.push_back(pd);
+      }
+    } else if (g.kind == qs::Gate::kDiag) {
+      Pend pd{};
+      pd.r.kind = REV_DIAG;
+      pd.r.axis = (uint32_t)g.na;
+      pd.r.moff = (uint32_t)vals.size();
+      for (const qs::cd &v : g.m) put(v);
+      pd.cmask = g.cmask;
+      pd.cval = g.cval;
+      pd.q0 = g.act[0];
+      pd.q1 = g.na > 1 ? g.act[1] : -1;
+      pend.push_back(pd);
+    } else {
+      Pend pd{};
+      pd.r.kind = REV_U1;
+      pd.r.moff = (uint32_t)vals.size();
+      for (const qs::cd &v : g.m) put(v);
+      pd.cmask = g.cmask;
+      pd.cval = g.cval;
+      pd.q0 = g.act[0];
+      pd.q1 = -1;
+      pend.push_back(pd);
+    }
+  }
+  return flush();
+}
+
 template <typename R>
 int backward_t(qs_adjoint *c, double *grads) {
   using T = typename AV2<R>::T;
@@ -636,7 +1097,12 @@ int backward_t(qs_adjoint *c, double *grads) {
   const uint64_t per = std::min<uint64_t>(pps, 1024);
   const int nblk = (int)((pps + per - 1) / per);
   int ck = (int)c->ckpt.size() - 1;
-  for (size_t ii = c->circ.ops.size(); ii-- > 0;) {
+  c->rev_blocks = c->rev_restores = 0;
+  if (adjoint_fused_reverse()) {
+    const int rc = reverse_blocks<R>(c);
+    if (rc != QS_OK) return rc;
+  }
+  for (size_t ii = adjoint_fused_reverse() ? 0 : c->circ.ops.size(); ii-- > 0;) {
     if (ck >= 0 && c->ckpt_at[ck] == ii + 1) {
       AJ(hipMemcpyAsync(c->psi, c->ckpt[ck], c->state_bytes(), hipMemcpyDeviceToDevice,
                         c->stream));
@@ -781,6 +1247,13 @@ int qs_adjoint_stats(const qs_adjoint *ctx, uint64_t *sweeps, uint64_t *kernels)
   if (!ctx || !sweeps || !kernels) return fail_arg("NULL argument");
   *sweeps = ctx->sweeps;
   *kernels = ctx->kernels;
+  return QS_OK;
+}
+
+int qs_adjoint_reverse_stats(const qs_adjoint *ctx, uint64_t *blocks, uint64_t *restores) {
+  if (!ctx || !blocks || !restores) return fail_arg("NULL argument");
+  *blocks = ctx->rev_blocks;
+  *restores = ctx->rev_restores;
   return QS_OK;
 }
 

@@ -99,6 +99,83 @@ def test_adjoint_vs_fd_and_parameter_shift(checkpoint):
         assert abs(ps - g[1, p]) < 1e-11
 
 
+def _mixed_circuit(n, seed):
+    """Rotations (X, Y, Z; some parameters used twice) between fixed gates of
+    every kind the reverse pass handles: H and CNOT (one dense target, the
+    CNOT's control off a 10-qubit tile), a controlled phase and a CZ (diagonal
+    on any qubits), a SWAP and a Haar 2-qubit gate (two dense targets: run
+    by the per-gate kernels between tiled blocks).  Returns (ops, nparams):
+    op = ("rot", axis, q, p) or ("gate", matrix, targets)."""
+    from tests.conftest import random_unitary
+    rng = np.random.default_rng(seed)
+    out, p = [], 0
+    for layer in range(3):
+        for q in rng.permutation(n)[:8]:
+            out.append(("rot", "xyz"[int(rng.integers(3))], int(q), p))
+            p += 1
+        out.append(("rot", "y", int(rng.integers(n)), p - 3))  # a shared parameter
+        g = ops.hadamard(int(rng.integers(n)))
+        out.append(("gate", g.matrix(), g.targets()))
+        g = ops.cnot(n - 1, 2)
+        out.append(("gate", g.matrix(), g.targets()))
+        cp = np.diag([1, 1, 1, np.exp(0.7j)])
+        out.append(("gate", cp, [3, n - 2]))
+        g = ops.cz(0, n - 3)
+        out.append(("gate", g.matrix(), g.targets()))
+        g = ops.swap(1, 9)
+        out.append(("gate", g.matrix(), g.targets()))
+        out.append(("gate", random_unitary(rng, 4), [5, n - 1]))
+        for q in rng.permutation(n)[:6]:
+            out.append(("rot", "xyz"[int(rng.integers(3))], int(q), p))
+            p += 1
+    return out, p
+
+
+def _mixed_oracle_energy(n, circ, angles, masks, coeffs):
+    """angles[i]: the angle of the i-th rotation occurrence."""
+    s = oracle.new([False] * n)
+    k = 0
+    for op in circ:
+        if op[0] == "rot":
+            g = {"x": ops.rx, "y": ops.ry, "z": ops.rz}[op[1]](angles[k], op[2])
+            k += 1
+            s = oracle.apply_fast(s, g.matrix(), g.targets())
+        else:
+            s = oracle.apply_fast(s, op[1], op[2])
+    return oracle.expect_zstrings(s, masks, coeffs)
+
+
+@pytest.mark.parametrize("dtype,tol", [("c128", 1e-11), ("c64", 2e-5)])
+def test_mixed_gates_shared_parameters_vs_parameter_shift(dtype, tol):
+    n = 13
+    circ, nparams = _mixed_circuit(n, seed=21)
+    masks, coeffs = circuits.zz_chain_observable(n)
+    th = circuits.seeded_thetas(2, nparams, seed=4)
+    pc = ParamCircuit(n)
+    for op in circ:
+        if op[0] == "rot":
+            pc.rotation(op[1], op[2], op[3])
+        else:
+            pc.gate(op[1], op[2])
+        pc.layer()
+    ex = Expectation(pc, masks, coeffs, dtype=dtype, checkpoint_every=4)
+    e = ex.forward(th)
+    g = ex.backward()
+    occ = [op[3] for op in circ if op[0] == "rot"]
+    for b in range(2):
+        ang = np.array([th[b, p] for p in occ])
+        assert abs(e[b] - _mixed_oracle_energy(n, circ, ang, masks, coeffs)) < tol
+        # parameter shift per occurrence, summed over a parameter's uses
+        ref = np.zeros(nparams)
+        for i, p in enumerate(occ):
+            ap, am = ang.copy(), ang.copy()
+            ap[i] += math.pi / 2
+            am[i] -= math.pi / 2
+            ref[p] += 0.5 * (_mixed_oracle_energy(n, circ, ap, masks, coeffs)
+                             - _mixed_oracle_energy(n, circ, am, masks, coeffs))
+        assert np.max(np.abs(g[b] - ref)) < tol
+
+
 def test_c64_adjoint_agrees_with_c128():
     n, layers, batch = 10, 4, 4
     ans = circuits.hardware_efficient_ansatz(n, layers)
