@@ -1357,25 +1357,50 @@ struct PosList {
   uint8_t p[64];
 };
 
-template <typename R>
+// Histogram of |psi|^2 over the 2^m patterns of the qubits po.p[0..m):
+// block b walks the free indices j of its chunk in order; for each j (the
+// same for the whole block, so expand() runs on scalar registers) thread t
+// loads the amplitudes of its BPT bins (h = t + 256 k) at once, so every
+// thread keeps BPT loads in flight.  With fewer than 256 bins, the threads
+// of bin h split the chunk S = 256 / 2^m ways (thread t: phase t / 2^m) and
+// each phase is a row of its own.  part[(b * S + phase) * 2^m + h].
+template <typename R, int BPT>
 __global__ __launch_bounds__(kBlock) void k_set_hist(const typename V2<R>::T *__restrict__ psi,
                                                      Fixed fx, PosList po, int m, uint64_t nfree,
                                                      uint64_t per_block,
                                                      double *__restrict__ part) {
   const int H = 1 << m;
+  const int S = H >= kBlock ? 1 : kBlock / H;
+  const int phase = H >= kBlock ? 0 : (int)threadIdx.x / H;
+  uint64_t dep[BPT];
+  double acc[BPT];
+#pragma unroll
+  for (int k = 0; k < BPT; ++k) {
+    const int h = H >= kBlock ? (int)threadIdx.x + k * kBlock : (int)threadIdx.x % H;
+    uint64_t d = 0;
+    for (int b = 0; b < m; ++b)
+      if ((h >> b) & 1) d |= 1ull << po.p[b];
+    dep[k] = d;
+    acc[k] = 0.0;
+  }
   const uint64_t j0 = (uint64_t)blockIdx.x * per_block;
   uint64_t j1 = j0 + per_block;
   if (j1 > nfree) j1 = nfree;
-  for (int h = threadIdx.x; h < H; h += kBlock) {
-    uint64_t dep = 0;
-    for (int j = 0; j < m; ++j)
-      if ((h >> j) & 1) dep |= 1ull << po.p[j];
-    double acc = 0.0;
-    for (uint64_t j = j0; j < j1; ++j) {
-      const auto a = psi[expand(j, fx) | dep];
-      acc += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
-    }
-    part[(size_t)blockIdx.x * H + h] = acc;
+  for (uint64_t jj = j0; jj < j1; jj += (uint64_t)S) {
+    const uint64_t j = jj + (uint64_t)phase;
+    if (j >= j1) break;
+    const uint64_t e = expand(j, fx);
+    typename V2<R>::T a[BPT];
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) a[k] = psi[e | dep[k]];
+#pragma unroll
+    for (int k = 0; k < BPT; ++k)
+      acc[k] += (double)a[k].x * (double)a[k].x + (double)a[k].y * (double)a[k].y;
+  }
+#pragma unroll
+  for (int k = 0; k < BPT; ++k) {
+    const int h = H >= kBlock ? (int)threadIdx.x + k * kBlock : (int)threadIdx.x % H;
+    part[((size_t)blockIdx.x * S + phase) * H + h] = acc[k];
   }
 }
 
@@ -1389,19 +1414,29 @@ __global__ __launch_bounds__(kBlock) void k_hist_finish(const double *__restrict
   }
 }
 
+// d_part holds blocks * max(1, 256 / 2^m) rows of 2^m partial sums.
 template <typename R>
 hipError_t launch_set_histogram(const void *psi, int n, const Fixed &fx, const uint8_t *pos_order,
                                 int m, double *d_part, int blocks, double *d_hist,
                                 hipStream_t s) {
   const int H = 1 << m;
+  if (m > 12) return hipErrorInvalidValue;
   const uint64_t nfree = 1ull << (n - m);
   const uint64_t per_block = (nfree + blocks - 1) / blocks;
   PosList po{};
   for (int j = 0; j < m; ++j) po.p[j] = pos_order[j];
-  hipLaunchKernelGGL(k_set_hist<R>, dim3(blocks), dim3(kBlock), 0, s,
-                     (const typename V2<R>::T *)psi, fx, po, m, nfree, per_block, d_part);
+  const int bpt = H >= kBlock ? H / kBlock : 1;
+  const int rows = blocks * (H >= kBlock ? 1 : kBlock / H);
+  using T = typename V2<R>::T;
+  auto kern = bpt == 1 ? k_set_hist<R, 1>
+              : bpt == 2 ? k_set_hist<R, 2>
+              : bpt == 4 ? k_set_hist<R, 4>
+              : bpt == 8 ? k_set_hist<R, 8>
+                         : k_set_hist<R, 16>;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, (const T *)psi, fx, po, m, nfree,
+                     per_block, d_part);
   hipLaunchKernelGGL(k_hist_finish, dim3((H + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     (const double *)d_part, blocks, H, d_hist);
+                     (const double *)d_part, rows, H, d_hist);
   return hipGetLastError();
 }
 

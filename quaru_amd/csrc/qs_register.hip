@@ -686,9 +686,10 @@ int measure_fast(qs_register *r, const size_t *order, size_t count, const double
     qs::Fixed fx = qs::make_fixed(smask, 0);
     const uint64_t nfree = 1ull << (n - m);
     int blocks = (int)std::min<uint64_t>(1024, nfree);
+    const size_t rows = (size_t)blocks * (H >= 256 ? 1 : 256 / H);  // of k_set_hist
     double *d_work = nullptr;
-    QS_HIP(ensure_big(r, ((size_t)blocks + 1) * H * sizeof(double), &d_work));
-    double *d_part = d_work, *d_hist = d_work + (size_t)blocks * H;
+    QS_HIP(ensure_big(r, (rows + 1) * H * sizeof(double), &d_work));
+    double *d_part = d_work, *d_hist = d_work + rows * H;
     uint8_t pos_order[64];
     for (int j = 0; j < m; ++j) pos_order[j] = (uint8_t)order[j];
     hipError_t e = qs::launch_set_histogram<R>(r->psi, n, fx, pos_order, m, d_part, blocks,
