@@ -291,6 +291,81 @@ struct RevHeader {
   uint32_t tq[kRevMaxTile];  // tile qubits, ascending
 };
 
+// One reverse step of a rotation on the pair (x0, x1) of psi and (l0, l1) of
+// lambda: acc += Im(conj(l0) s0 + conj(l1) s1) with s = sigma . psi, then
+// U^dagger = [[conj m0, conj m2], [conj m1, conj m3]] on both pairs.
+template <typename R>
+__device__ __forceinline__ void rot_reverse_pair(typename AV2<R>::T &x0, typename AV2<R>::T &x1,
+                                                 typename AV2<R>::T &l0, typename AV2<R>::T &l1,
+                                                 uint32_t axis, const typename AV2<R>::T *m,
+                                                 double &acc) {
+  using T = typename AV2<R>::T;
+  double s0r, s0i, s1r, s1i;
+  if (axis == 0) {
+    s0r = x1.x; s0i = x1.y; s1r = x0.x; s1i = x0.y;
+  } else if (axis == 1) {
+    s0r = x1.y; s0i = -(double)x1.x; s1r = -(double)x0.y; s1i = x0.x;
+  } else {
+    s0r = x0.x; s0i = x0.y; s1r = -(double)x1.x; s1i = -(double)x1.y;
+  }
+  acc += (double)l0.x * s0i - (double)l0.y * s0r + (double)l1.x * s1i - (double)l1.y * s1r;
+  const T m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+  T y0, y1, k0, k1;
+  y0.x = m0.x * x0.x + m0.y * x0.y + m2.x * x1.x + m2.y * x1.y;
+  y0.y = m0.x * x0.y - m0.y * x0.x + m2.x * x1.y - m2.y * x1.x;
+  y1.x = m1.x * x0.x + m1.y * x0.y + m3.x * x1.x + m3.y * x1.y;
+  y1.y = m1.x * x0.y - m1.y * x0.x + m3.x * x1.y - m3.y * x1.x;
+  k0.x = m0.x * l0.x + m0.y * l0.y + m2.x * l1.x + m2.y * l1.y;
+  k0.y = m0.x * l0.y - m0.y * l0.x + m2.x * l1.y - m2.y * l1.x;
+  k1.x = m1.x * l0.x + m1.y * l0.y + m3.x * l1.x + m3.y * l1.y;
+  k1.y = m1.x * l0.y - m1.y * l0.x + m3.x * l1.y - m3.y * l1.x;
+  x0 = y0;
+  x1 = y1;
+  l0 = k0;
+  l1 = k1;
+}
+
+// Sum over the 64 lanes of a wave, every lane receiving the total: an xor
+// butterfly on VALU cross-lane moves (DPP within rows of 16, gfx950's
+// permlane swaps across them) instead of LDS-crossbar shuffles.
+__device__ __forceinline__ int dpp_i(int v, int ctrl) {
+  switch (ctrl) {
+    case 0: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    default: return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false); // row_mirror
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  // steps 1, 2: lane ^ 1, lane ^ 2; then every quad is uniform, so the half
+  // mirror adds the other quad of each 8 and the row mirror the other 8
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i((int)(b & 0xffffffffll), c), hi = dpp_i((int)(b >> 32), c);
+    v += __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+  // rows: every row of 16 is uniform; the swaps hand each lane the value of
+  // the row 16 (then 32) lanes away
+  const long long b16 = __double_as_longlong(v);
+  const auto l16 = __builtin_amdgcn_permlane16_swap((int)(b16 & 0xffffffffll),
+                                                    (int)(b16 & 0xffffffffll), false, false);
+  const auto h16 = __builtin_amdgcn_permlane16_swap((int)(b16 >> 32), (int)(b16 >> 32), false,
+                                                    false);
+  const bool r16 = (threadIdx.x >> 4) & 1;
+  const int lo16 = r16 ? l16[0] : l16[1], hi16 = r16 ? h16[0] : h16[1];
+  v += __longlong_as_double(((long long)hi16 << 32) | (unsigned int)lo16);
+  const long long b32 = __double_as_longlong(v);
+  const auto l32 = __builtin_amdgcn_permlane32_swap((int)(b32 & 0xffffffffll),
+                                                    (int)(b32 & 0xffffffffll), false, false);
+  const auto h32 = __builtin_amdgcn_permlane32_swap((int)(b32 >> 32), (int)(b32 >> 32), false,
+                                                    false);
+  const bool r32 = (threadIdx.x >> 5) & 1;
+  const int lo32 = r32 ? l32[0] : l32[1], hi32 = r32 ? h32[0] : h32[1];
+  return v + __longlong_as_double(((long long)hi32 << 32) | (unsigned int)lo32);
+}
+
 // One workgroup per tile (grid = ntiles); dynamic LDS 2 x 2^T amplitudes;
 // PER = max(1, 2^T / 256).  part[out * ntiles + tile] = the tile's share of
 // rotation `out`'s gradient.
@@ -303,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tile(typename AV2<R>::T *__r
                                                          uint64_t ntiles) {
   using T = typename AV2<R>::T;
   extern __shared__ unsigned char rev_smem[];
-  __shared__ double red[2][kBlock / 64];
+  __shared__ double red[4][kBlock / 64];
   const RevHeader &h = *(const RevHeader *)blob;
   const RevRec *recs = (const RevRec *)(blob + sizeof(RevHeader));
   const R *vals = (const R *)(blob + sizeof(RevHeader) + h.nops * sizeof(RevRec));
@@ -353,51 +428,76 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tile(typename AV2<R>::T *__r
   for (uint32_t o = 0; o < h.nops; ++o) {
     const RevRec &r = recs[o];
     if (r.kind == REV_ROT) {
+      // two consecutive rotations on different tile bits run in one pass
+      // over quads of amplitudes; rows use four buffers by row number, so a
+      // buffer is rewritten only after a barrier that thread 0 passes after
+      // reading it
+      const bool two = o + 1 < h.nops && recs[o + 1].kind == REV_ROT && recs[o + 1].k != r.k;
+      const RevRec &r2 = two ? recs[o + 1] : r;
       const T *mm = rot + ((uint64_t)r.rslot * Bp + b) * 4;
-      const T m0 = mm[0], m1 = mm[1], m2 = mm[2], m3 = mm[3];
-      const uint32_t bk = 1u << r.k;
-      double acc = 0.0;
+      const T *mm2 = rot + ((uint64_t)r2.rslot * Bp + b) * 4;
+      const T m[4] = {mm[0], mm[1], mm[2], mm[3]};
+      const T n2[4] = {mm2[0], mm2[1], mm2[2], mm2[3]};
+      double acc = 0.0, acc2 = 0.0;
+      if (two) {
+        const uint32_t b1 = 1u << r.k, b2 = 1u << r2.k;
+        const int klo = r.k < r2.k ? (int)r.k : (int)r2.k;
+        const int khi = r.k < r2.k ? (int)r2.k : (int)r.k;
 #pragma unroll
-      for (int pi = 0; pi < (PER + 1) / 2; ++pi) {
-        const uint32_t p = threadIdx.x + (uint32_t)pi * kBlock;
-        if (p >= E / 2) break;
-        const uint32_t j0 = (uint32_t)insert0(p, (int)r.k), j1 = j0 | bk;
-        const T x0 = ps[j0], x1 = ps[j1], l0 = ls[j0], l1 = ls[j1];
-        double s0r, s0i, s1r, s1i;  // sigma . psi
-        if (r.axis == 0) {
-          s0r = x1.x; s0i = x1.y; s1r = x0.x; s1i = x0.y;
-        } else if (r.axis == 1) {
-          s0r = x1.y; s0i = -(double)x1.x; s1r = -(double)x0.y; s1i = x0.x;
-        } else {
-          s0r = x0.x; s0i = x0.y; s1r = -(double)x1.x; s1i = -(double)x1.y;
+        for (int qi = 0; qi < (PER + 3) / 4; ++qi) {
+          const uint32_t q = threadIdx.x + (uint32_t)qi * kBlock;
+          if (q >= E / 4) break;
+          const uint32_t j00 = (uint32_t)insert0(insert0(q, klo), khi);
+          const uint32_t j01 = j00 | b1, j10 = j00 | b2, j11 = j00 | b1 | b2;
+          T x00 = ps[j00], x01 = ps[j01], x10 = ps[j10], x11 = ps[j11];
+          T l00 = ls[j00], l01 = ls[j01], l10 = ls[j10], l11 = ls[j11];
+          rot_reverse_pair<R>(x00, x01, l00, l01, r.axis, m, acc);
+          rot_reverse_pair<R>(x10, x11, l10, l11, r.axis, m, acc);
+          rot_reverse_pair<R>(x00, x10, l00, l10, r2.axis, n2, acc2);
+          rot_reverse_pair<R>(x01, x11, l01, l11, r2.axis, n2, acc2);
+          ps[j00] = x00;
+          ps[j01] = x01;
+          ps[j10] = x10;
+          ps[j11] = x11;
+          ls[j00] = l00;
+          ls[j01] = l01;
+          ls[j10] = l10;
+          ls[j11] = l11;
         }
-        acc += (double)l0.x * s0i - (double)l0.y * s0r + (double)l1.x * s1i - (double)l1.y * s1r;
-        // U^dagger = [[conj m0, conj m2], [conj m1, conj m3]]
-        T y0, y1, k0, k1;
-        y0.x = m0.x * x0.x + m0.y * x0.y + m2.x * x1.x + m2.y * x1.y;
-        y0.y = m0.x * x0.y - m0.y * x0.x + m2.x * x1.y - m2.y * x1.x;
-        y1.x = m1.x * x0.x + m1.y * x0.y + m3.x * x1.x + m3.y * x1.y;
-        y1.y = m1.x * x0.y - m1.y * x0.x + m3.x * x1.y - m3.y * x1.x;
-        k0.x = m0.x * l0.x + m0.y * l0.y + m2.x * l1.x + m2.y * l1.y;
-        k0.y = m0.x * l0.y - m0.y * l0.x + m2.x * l1.y - m2.y * l1.x;
-        k1.x = m1.x * l0.x + m1.y * l0.y + m3.x * l1.x + m3.y * l1.y;
-        k1.y = m1.x * l0.y - m1.y * l0.x + m3.x * l1.y - m3.y * l1.x;
-        ps[j0] = y0;
-        ps[j1] = y1;
-        ls[j0] = k0;
-        ls[j1] = k1;
+      } else {
+        const uint32_t bk = 1u << r.k;
+#pragma unroll
+        for (int pi = 0; pi < (PER + 1) / 2; ++pi) {
+          const uint32_t p = threadIdx.x + (uint32_t)pi * kBlock;
+          if (p >= E / 2) break;
+          const uint32_t j0 = (uint32_t)insert0(p, (int)r.k), j1 = j0 | bk;
+          T x0 = ps[j0], x1 = ps[j1], l0 = ls[j0], l1 = ls[j1];
+          rot_reverse_pair<R>(x0, x1, l0, l1, r.axis, m, acc);
+          ps[j0] = x0;
+          ps[j1] = x1;
+          ls[j0] = l0;
+          ls[j1] = l1;
+        }
       }
-      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-      // two buffers by row parity: a buffer is rewritten two rotations
-      // later, after the barrier of the rotation between
-      double *rb = red[r.out & 1u];
+      acc = wave_sum(acc);
+      double *rb = red[r.out & 3u];
+      double *rb2 = red[r2.out & 3u];
       if ((threadIdx.x & 63u) == 0) rb[threadIdx.x >> 6] = acc;
+      if (two) {
+        acc2 = wave_sum(acc2);
+        if ((threadIdx.x & 63u) == 0) rb2[threadIdx.x >> 6] = acc2;
+      }
       __syncthreads();
       if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) t += rb[w];
+        double t = 0.0, t2 = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+          t += rb[w];
+          t2 += rb2[w];
+        }
         part[(uint64_t)r.out * ntiles + tile] = t;
+        if (two) part[(uint64_t)r2.out * ntiles + tile] = t2;
       }
+      if (two) ++o;
     } else if (r.kind == REV_PHASE || r.kind == REV_DIAG) {
       // a run of consecutive phase and diagonal records: one LDS pass, each
       // amplitude multiplied by every factor whose pattern it meets
