@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <type_traits>
 #include <vector>
 
@@ -462,7 +463,8 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
         const Gate &g = gates[gi];
         cd u[4];
         const uint64_t touch = gx[gi] | gz[gi];
-        if (plain_2x2(g, u) && !(later_touch & (1ull << g.act[0]))) {
+        const bool bat = is_batched_2x2(g);
+        if ((bat || plain_2x2(g, u)) && !(later_touch & (1ull << g.act[0]))) {
           const uint64_t bt = 1ull << g.act[0];
           // the next gate on t outside this sweep
           for (size_t i = (size_t)gi + 1; i < G; ++i) {
@@ -471,10 +473,18 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
             int tt;
             uint64_t pm, pv;
             cd mt[4];
-            bool mergeable = plain_2x2(gates[i], mt);
-            for (int choice = 0; choice < 8 && !mergeable; ++choice) {
-              if (!pattern_2x2(gates[i], choice, tt, pm, pv, mt)) break;
-              mergeable = tt == g.act[0] && popc(pm) <= 2;
+            bool mergeable;
+            if (bat) {
+              // a batched 2x2 merges only with another one on t (merge_batched)
+              const Gate &h = gates[i];
+              mergeable = is_batched_2x2(h) && h.act[0] == g.act[0] && h.bshift == g.bshift &&
+                          h.bmask == g.bmask;
+            } else {
+              mergeable = plain_2x2(gates[i], mt);
+              for (int choice = 0; choice < 8 && !mergeable; ++choice) {
+                if (!pattern_2x2(gates[i], choice, tt, pm, pv, mt)) break;
+                mergeable = tt == g.act[0] && popc(pm) <= 2;
+              }
             }
             if (mergeable) drop[k] = 1;
             break;
@@ -571,7 +581,11 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     for (size_t k = 0; k < pl.gates.size(); ++k) items[k].g = &gates[pl.gates[k]];
     uint64_t reg_qubits = 0;
     for (int j = 0; j < A; ++j) reg_qubits |= 1ull << regs[j];
-    if (sweep_merge_ops()) merge_sweep_ops(items, slot, reg_qubits);
+    std::deque<Gate> merged_store;
+    if (sweep_merge_ops()) {
+      merge_batched(items, merged_store);
+      merge_sweep_ops(items, slot, reg_qubits);
+    }
     for (const SweepOp &o : items) {
       if (!o.alive) {
         if (run.merged) ++*run.merged;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <deque>
 #include <vector>
 
 #include "qs_internal.h"
@@ -146,6 +147,50 @@ inline bool mux_encodable(int t, uint64_t pm, const int *slot, uint64_t reg_qubi
   return mux_form(t, pm, slot, reg_qubits) != 0;
 }
 
+// Qubits a gate touches in any way (targets, controls, phase patterns).
+inline uint64_t touched(const Gate &g) {
+  uint64_t m = g.cmask | g.smask;
+  for (int j = 0; j < g.na; ++j) m |= 1ull << g.act[j];
+  return m;
+}
+
+// Batched 2x2s (one matrix per state of a batch, the adjoint's rotations) on
+// the same qubit t with nothing in between that touches t: the later one is
+// folded into the earlier, state by state (M2_b M1_b), so the sweep runs one
+// batched op where it ran two.  The merged gates live in `store` (stable
+// addresses for the items that point at them).
+inline bool is_batched_2x2(const Gate &g) {
+  return g.kind == Gate::kDense && g.na == 1 && g.bmask != 0 && g.cmask == 0;
+}
+
+inline void merge_batched(std::vector<SweepOp> &items, std::deque<Gate> &store) {
+  auto batched = [](const Gate *g) { return g && is_batched_2x2(*g); };
+  for (size_t i = 0; i < items.size(); ++i) {
+    if (!items[i].alive || !batched(items[i].g)) continue;
+    const int t = items[i].g->act[0];
+    const uint64_t bt = 1ull << t;
+    for (size_t k = i + 1; k < items.size(); ++k) {
+      if (!items[k].alive) continue;
+      const Gate *h = items[k].g;
+      if (h == nullptr) break;
+      if (!(touched(*h) & bt)) continue;
+      if (batched(h) && h->act[0] == t && h->bshift == items[i].g->bshift &&
+          h->bmask == items[i].g->bmask) {
+        const Gate &a = *items[i].g;
+        Gate mg = a;
+        const size_t nb = a.m.size() / 4;
+        for (size_t b = 0; b < nb; ++b) mul2(&h->m[4 * b], &a.m[4 * b], &mg.m[4 * b]);
+        mg.logical = a.logical + h->logical;
+        store.push_back(std::move(mg));
+        items[i].g = &store.back();
+        items[k].alive = false;
+        continue;  // the merged op may absorb the next one on t as well
+      }
+      break;  // another gate on t
+    }
+  }
+}
+
 inline void merge_sweep_ops(std::vector<SweepOp> &items, const int *slot, uint64_t reg_qubits) {
   int last[64], lastx[64];  // last op touching q at all / non-diagonally
   for (int q = 0; q < 64; ++q) last[q] = lastx[q] = -1;
@@ -159,6 +204,7 @@ inline void merge_sweep_ops(std::vector<SweepOp> &items, const int *slot, uint64
   };
   for (int i = 0; i < (int)items.size(); ++i) {
     SweepOp &G = items[i];
+    if (!G.alive) continue;  // folded into an earlier batched op
     cd u[4], uj[4], mt[4];
     bool merged = false;
     if (is_plain(G, u)) {
