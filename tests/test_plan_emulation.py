@@ -109,8 +109,9 @@ def test_batched_rotations_plan(tmp_path):
     """The adjoint's fused forward pass: a batch of 8 states stacked in index
     bits 13..15 of a 16-qubit register, layers of per-state RY/RZ rotations
     (batched 2x2s: the kernel picks state b's matrix by the tile's batch
-    bits) and a CZ ring.  The planner must keep the batch bits off the tile
-    and every state must equal its own circuit on the oracle."""
+    bits) and a CZ ring.  The planner must keep the batch bits off the tile,
+    fold each RZ(q) into the RY(q) before it, and every state must equal its
+    own circuit on the oracle."""
     out = str(tmp_path / "libplan_emulator.so")
     subprocess.run([HIPCC, "-x", "hip", "--offload-arch=gfx950", "-std=c++17", "-O1", "-fPIC",
                     "-shared", "-I", CSRC, SRC, "-L", LIBDIR, "-lquaru_hip",
@@ -160,6 +161,8 @@ def test_batched_rotations_plan(tmp_path):
                           bm_a.ctypes.data, n, stats.ctypes.data, None, err, 512)
         assert rc == 0, err.value.decode()
         assert stats[0] >= 2 and stats[1] == 0, stats  # sweeps only
+        # every RZ(q) folds into the RY(q) before it (state by state)
+        assert stats[3] == layers * n, stats
         for b in range(B):
             ref = np.zeros(1 << n, dtype=np.complex128)
             ref[0] = 1.0
