@@ -101,6 +101,20 @@ def run_steps(reg: Register, opl, steps: int) -> float:
     return time.perf_counter() - t0
 
 
+def jit_wait():
+    """Wait for the sweep kernels specialised during the warmup (compiled on
+    host threads while the warmup ran the generic kernel) so that the timed
+    steps run them; logs the totals."""
+    import ctypes
+    lib = _lib.load()
+    t0 = time.perf_counter()
+    lib.qs_sweep_jit_wait()
+    c, f, n = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    lib.qs_sweep_jit_stats(ctypes.byref(c), ctypes.byref(f), ctypes.byref(n))
+    log(f"specialised sweep kernels: {c.value} compiled, {f.value} failed "
+        f"(waited {time.perf_counter() - t0:.1f} s)")
+
+
 def timed_run(reg: Register, opl, steps: int, warmup: int):
     """Warm up, then time `steps` passes of `opl` with the kernel timers on.
     Returns (seconds, per-class kernel ms, per-class launches, stats)."""
@@ -109,6 +123,8 @@ def timed_run(reg: Register, opl, steps: int, warmup: int):
             reg.apply(op)
         reg.sync()
         log(f"warmup {i + 1}/{warmup} done")
+    if warmup:
+        jit_wait()
     reg.stats(reset=True)
     timing(reg)
     timing_enable(reg, True)
@@ -207,6 +223,7 @@ def adjoint_bench(n: int, layers: int, batch: int, fd_params: int, dtype: str):
     ex = Expectation(pc, masks, coeffs, dtype=dtype, checkpoint_every=ckpt)
     ex.forward(th)
     ex.backward()  # warmup
+    jit_wait()
     device_sync()
     t0 = time.perf_counter()
     _, run = ex.record(th)

@@ -278,6 +278,20 @@ int qs_adjoint_backward(qs_adjoint *ctx, double *grads);
  * reference counterpart; diagnostics). */
 int qs_adjoint_stats(const qs_adjoint *ctx, uint64_t *sweeps, uint64_t *kernels);
 
+/* Sweep kernels specialised to one op-code sequence, compiled at run time
+ * with hipRTC (no reference counterpart; engine tuning).  mode 0: off; 1
+ * (default, or QS_SWEEP_JIT): a new sequence runs the generic kernel while
+ * its specialised kernel compiles on host threads; 2: compile before the
+ * first launch.  mode < 0 only queries.  Returns the previous mode. */
+int qs_sweep_jit_mode(int mode);
+
+/* Blocks until every queued compile has finished. */
+int qs_sweep_jit_wait(void);
+
+/* Kernels compiled, compiles failed (those sequences stay on the generic
+ * kernel), and launches of compiled kernels, since the process started. */
+int qs_sweep_jit_stats(uint64_t *compiled, uint64_t *failed, uint64_t *launches);
+
 /* Tiled passes over psi and lambda, and checkpoint restores, of the last
  * backward of *ctx (no reference counterpart; diagnostics). */
 int qs_adjoint_reverse_stats(const qs_adjoint *ctx, uint64_t *blocks, uint64_t *restores);

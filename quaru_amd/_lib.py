@@ -82,6 +82,9 @@ SIGNATURES = {
     "qs_adjoint_backward": (_I, [_P, _P]),
     "qs_adjoint_stats": (_I, [_P, _P, _P]),
     "qs_adjoint_reverse_stats": (_I, [_P, _P, _P]),
+    "qs_sweep_jit_mode": (_I, [_I]),
+    "qs_sweep_jit_wait": (_I, []),
+    "qs_sweep_jit_stats": (_I, [_P, _P, _P]),
     "qs_adjoint_free": (None, [_P]),
 }
 

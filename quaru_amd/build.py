@@ -17,7 +17,8 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libquaru_hip.so")
-SOURCES = ["qs_kernels.hip", "qs_sweep.hip", "qs_register.hip", "qs_adjoint.hip"]
+SOURCES = ["qs_kernels.hip", "qs_sweep.hip", "qs_register.hip", "qs_adjoint.hip",
+           "qs_sweep_jit.hip"]
 # qs_sweep_launch.hip holds the k_sweep instantiations of one (precision,
 # register bits) pair; it is compiled once per pair so that they build in
 # parallel.
@@ -60,6 +61,23 @@ def _stale(obj: str, src: str) -> bool:
     return os.path.getmtime(obj) <= max(os.path.getmtime(d) for d in deps)
 
 
+def write_kernel_source(objdir: str) -> str:
+    """The sweep kernel header as a C++ string literal
+    (qs_sweep_kernel_src.inc), embedded in the library for the run-time
+    compiles of specialised sweep kernels (qs_sweep_jit.hip)."""
+    with open(os.path.join(CSRC, "qs_sweep_kernel.h")) as f:
+        text = f.read()
+    if ")QSK\"" in text:
+        raise RuntimeError("qs_sweep_kernel.h contains the raw-string delimiter")
+    out = os.path.join(objdir, "qs_sweep_kernel_src.inc")
+    body = 'R"QSK(' + text + ')QSK"\n'
+    old = open(out).read() if os.path.exists(out) else None
+    if old != body:
+        with open(out, "w") as f:
+            f.write(body)
+    return out
+
+
 def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> str:
     """Compile each translation unit to an object in parallel, then link the
     library.  ``incremental`` keeps objects in quaru_amd/_build and recompiles
@@ -72,6 +90,7 @@ def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> st
     else:
         objdir = tempfile.mkdtemp(prefix="quaru_hip_build_")
     try:
+        write_kernel_source(objdir)
         procs = []
         objs = []
         for src, extra, oname in units():
@@ -80,7 +99,7 @@ def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> st
             objs.append(obj)
             if incremental and not _stale(obj, path):
                 continue
-            cmd = [cc, *HIPCC_FLAGS, *extra, "-c", path, "-o", obj]
+            cmd = [cc, *HIPCC_FLAGS, *extra, "-I", objdir, "-c", path, "-o", obj]
             if verbose:
                 print(" ".join(cmd), flush=True)
             procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,
@@ -98,7 +117,7 @@ def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> st
             msg = "\n".join(f"--- {s}\n{o}" for s, o in failed)
             raise RuntimeError(f"hipcc failed:\n{msg}")
         tmp = LIB + ".tmp"
-        cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+        cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lhiprtc", "-o", tmp]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -27,6 +27,7 @@
 
 #include "qs_sweep.h"
 #include "qs_sweep_kernel.h"
+#include "qs_sweep_jit.h"
 #include "qs_sweep_launch.h"
 #include "qs_sweep_merge.h"
 
@@ -283,8 +284,26 @@ void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, uint64_t
 }
 
 template <typename R>
-hipError_t launch_sweep_a(int A, int W, SweepRunner &run, const void *d_blob, uint64_t ntiles) {
+hipError_t launch_sweep_a(int A, int W, SweepRunner &run, const void *d_blob, uint64_t ntiles,
+                          const std::vector<OpRec> &ops) {
   const bool xcd = sweep_xcd_chunks();
+  // a kernel specialised to this op-code sequence, when one is compiled
+  // (32-bit indices only)
+  if (run.n <= 32 && jit_mode() != 0) {
+    const bool c64 = std::is_same<R, float>::value;
+    hipFunction_t fn = jit_kernel(jit_signature(ops, c64, A, W), ops, c64, A, W);
+    if (fn) {
+      uint64_t grid;
+      uint32_t xcd_chunk;
+      sweep_grid(W, ntiles, xcd, &grid, &xcd_chunk);
+      void *psi = run.psi;
+      const void *blob = d_blob;
+      void *args[] = {&psi, &blob, &xcd_chunk};
+      jit_count_launch();
+      return hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, kThreads, 1, 1, 0, run.stream, args,
+                                   nullptr);
+    }
+  }
   switch (A) {
     case 3: return launch_sweep_ra<R, 3>(W, run.n, run.stream, run.psi, d_blob, ntiles, xcd);
     case 4: return launch_sweep_ra<R, 4>(W, run.n, run.stream, run.psi, d_blob, ntiles, xcd);
@@ -618,7 +637,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     hipError_t e = run.upload(blob.data(), blob.size(), &d_blob);
     if (e != hipSuccess) return e;
     if (run.mark) run.mark(true);
-    e = launch_sweep_a<R>(A, W, run, d_blob, h.ntiles);
+    e = launch_sweep_a<R>(A, W, run, d_blob, h.ntiles, ops);
     if (run.mark) run.mark(false);
     if (e != hipSuccess) return e;
     ++*sweeps;

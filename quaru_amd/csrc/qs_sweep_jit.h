@@ -1,0 +1,48 @@
+// qs_sweep_jit.h -- sweep kernels specialised to one op list, compiled at
+// run time with hipRTC.
+//
+// The generic sweep kernel (k_sweep) reads each op's code from the
+// descriptor and dispatches through a switch: every op pays the dispatch and
+// a whole-tile register copy at the loop's back edge.  A specialised kernel
+// for a given sequence of op codes runs the same op functions (apply_code in
+// qs_sweep_kernel.h) as straight-line code; matrices, slots' qubits and
+// patterns still come from the descriptor, so one compiled kernel serves
+// every sweep with the same code sequence (a repeated circuit, the layers of
+// a circuit, the steps of an optimiser).
+//
+// Modes (QS_SWEEP_JIT, or qs_sweep_jit_mode): 0 off; 1 (default) a new code
+// sequence launches the generic kernel and queues a compile on a pool of
+// host threads, later sweeps with that sequence use the compiled kernel;
+// 2 compile before the first launch.  A failed compile leaves the sequence
+// on the generic kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace qs {
+namespace sweep {
+
+struct OpRec;
+
+// The key of a specialised kernel: precision, register bits, wave bits and
+// each op's code (and its fail code, and whether it is batched).
+std::string jit_signature(const std::vector<OpRec> &ops, bool c64, int A, int W);
+
+// The compiled kernel for `sig` on the current device, or nullptr (not
+// compiled yet, compiling, off or failed).  Queues the compile of a new
+// signature; in mode 2 waits for it.
+hipFunction_t jit_kernel(const std::string &sig, const std::vector<OpRec> &ops, bool c64, int A,
+                         int W);
+
+int jit_mode();
+int jit_set_mode(int mode);  // returns the previous mode
+void jit_wait();             // until every queued compile has finished
+void jit_stats(uint64_t *compiled, uint64_t *failed, uint64_t *launches);
+void jit_count_launch();
+
+}  // namespace sweep
+}  // namespace qs

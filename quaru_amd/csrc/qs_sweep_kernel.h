@@ -32,11 +32,18 @@
 // (complex structs selected as a whole would be spilled to scratch).
 #pragma once
 
+#ifdef __HIPCC_RTC__
+// compiled at run time (hipRTC) for one sweep's op list: the runtime's own
+// headers, fixed-width types from its namespace
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+#else
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
 
 #include "qs_internal.h"
+#endif
 
 #define QS_CONST __attribute__((address_space(4)))
 
@@ -552,13 +559,22 @@ __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpR
   const Pred<I> p = make_pred<A>(op, c, g);
   constexpr int bj = 1 << J;
   R lr[4], li[4];
+  // both matrices on scalar loads once; register patterns choose per pair
+  // with scalar selects (a load per pair would hold 2^(A-1) matrices in
+  // scalar registers at once)
+  R v0r[4], v0i[4], v1r[4], v1i[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v0r[k] = sld(m + 2 * k);
+    v0i[k] = sld(m + 2 * k + 1);
+    v1r[k] = sld(m + 8 + 2 * k);
+    v1i[k] = sld(m + 9 + 2 * k);
+  }
   if constexpr (!kRegPattern) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const R r0 = sld(m + 2 * k), i0 = sld(m + 2 * k + 1);
-      const R r1 = sld(m + 8 + 2 * k), i1 = sld(m + 9 + 2 * k);
-      lr[k] = p.lane_ok ? r1 : r0;
-      li[k] = p.lane_ok ? i1 : i0;
+      lr[k] = p.lane_ok ? v1r[k] : v0r[k];
+      li[k] = p.lane_ok ? v1i[k] : v0i[k];
     }
   }
 #pragma unroll
@@ -566,11 +582,11 @@ __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpR
     if ((r >> J) & 1) continue;
     R tr[4], ti[4];
     if constexpr (kRegPattern) {
-      const QS_CONST R *mm = ((goff<A>(g, r) & p.cmr) == p.cvr) ? m + 8 : m;
+      const bool hold = (goff<A>(g, r) & p.cmr) == p.cvr;  // the same for the whole wave
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        tr[k] = mm[2 * k];
-        ti[k] = mm[2 * k + 1];
+        tr[k] = hold ? v1r[k] : v0r[k];
+        ti[k] = hold ? v1i[k] : v0i[k];
       }
     } else {
 #pragma unroll
@@ -937,120 +953,106 @@ __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec 
   }
 }
 
-// Every op of the sweep on one wave's tile held in registers.
+// One op, its form fixed at compile time: CODE is the planner's op code
+// (type and register slots folded together, see op_code).  Codes naming a
+// register slot the kernel does not have (j >= A) or a wave bit of a
+// kernel without wave bits compile to nothing, like CODE_NONE.
+template <uint32_t CODE, int A, int W, typename R, typename I>
+__device__ __forceinline__ void apply_code(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                           const QS_CONST R *m, const Ctx<I> &c,
+                                           const Regs<A, I> &g) {
+  constexpr uint32_t C = CODE;
+  if constexpr (C >= CODE_U1 && C < CODE_U1 + kMaxA) {
+    if constexpr (C - CODE_U1 < (uint32_t)A) op_1q_reg<C - CODE_U1, false, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_U1 + kMaxA && C < CODE_C1) {
+    op_1q_lane<false, C - CODE_U1 - kMaxA, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_C1 && C < CODE_C1 + kMaxA) {
+    if constexpr (C - CODE_C1 < (uint32_t)A) op_1q_reg<C - CODE_C1, true, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_C1 + kMaxA && C < CODE_X1) {
+    op_1q_lane<true, C - CODE_C1 - kMaxA, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_X1 && C < CODE_X1 + kMaxA) {
+    if constexpr (C - CODE_X1 < (uint32_t)A) op_x1_reg<C - CODE_X1, A, R, I>(a, op, c, g);
+  } else if constexpr (C >= CODE_X1 + kMaxA && C < CODE_SPARSE) {
+    op_x1_lane<C - CODE_X1 - kMaxA, A, R, I>(a, op, c, g);
+  } else if constexpr (C == CODE_SPARSE) {
+    op_sparse<A>(a, op, m, c, g);
+  } else if constexpr (C == CODE_DIAG) {
+    op_diag<A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_DENSE2 && C < CODE_MUXL) {
+    constexpr int J0 = (int)((C - CODE_DENSE2) / kMaxA), J1 = (int)((C - CODE_DENSE2) % kMaxA);
+    if constexpr (J0 < A && J1 < A && J0 != J1) op_dense2<J0, J1, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_MUXL && C < CODE_MUXL + kMaxA) {
+    if constexpr (C - CODE_MUXL < (uint32_t)A) op_mux_reg<C - CODE_MUXL, false, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_MUXL + kMaxA && C < CODE_MUXR) {
+    op_mux_lane<C - CODE_MUXL - kMaxA, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_MUXR && C < CODE_MUXRL) {
+    if constexpr (C - CODE_MUXR < (uint32_t)A) op_mux_reg<C - CODE_MUXR, true, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_MUXRL && C < CODE_U1W) {
+    constexpr int JC = (int)((C - CODE_MUXRL) / kLaneBits), LK = (int)((C - CODE_MUXRL) % kLaneBits);
+    if constexpr (JC < A) op_mux_lane_reg<JC, LK, A>(a, op, m, c, g);
+  } else if constexpr (C >= CODE_U1W && C <= CODE_MUXW) {
+    if constexpr (W > 0) op_wave<(int)(C - CODE_U1W), A>(a, op, m, c, g);
+  } else if constexpr (C == CODE_NEG) {
+    op_neg<A, R, I>(a, op, c, g);
+  }
+}
+
+// The op's matrix pool entry and its code for this tile: pattern bits off
+// the tile (umask) choose between `code` (values at moff + uoff) and
+// `code_fail`; a batched op's matrix follows the tile's batch index.  The
+// tile base is the same for every lane of a wave: it is read from the first
+// lane, so that the choices stay scalar.
+template <typename R, typename I>
+__device__ __forceinline__ uint32_t op_setup(const QS_CONST OpRec &op, const QS_CONST R *pool,
+                                             const Ctx<I> &c, const QS_CONST R *&m) {
+  m = pool + op.moff;
+  uint32_t code = op.code;
+  if (op.umask) {
+    const uint32_t tok = __builtin_amdgcn_readfirstlane(
+        (uint32_t)((((uint64_t)c.tbase ^ op.cval) & op.umask) == 0));
+    if (tok)
+      m += op.uoff;
+    else
+      code = op.code_fail;
+  }
+  if (op.bmask)
+    m += 8 * __builtin_amdgcn_readfirstlane(
+                 (uint32_t)(((uint64_t)c.tbase >> op.bshift) & op.bmask));
+  return code;
+}
+
+// One op of the sweep on one wave's tile held in registers.
 //
 // One flat switch over the op's code (type and register slots folded
 // together by the planner, see op_code) rather than nested switches on type
 // and slot: each nesting level is a join where the compiler copied the whole
 // amplitude array between register sets, three whole-array copies per op.
 template <int A, int W, typename R, typename I>
+__device__ __forceinline__ void run_op(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
+                                       const QS_CONST R *pool, const Ctx<I> &c,
+                                       const Regs<A, I> &g) {
+  const QS_CONST R *m;
+  const uint32_t code = op_setup<R, I>(op, pool, c, m);
+#define QS_CASE(K) \
+  case (K): apply_code<(K), A, W>(a, op, m, c, g); break;
+#define QS_CASE8(K) QS_CASE(K) QS_CASE(K + 1) QS_CASE(K + 2) QS_CASE(K + 3) \
+  QS_CASE(K + 4) QS_CASE(K + 5) QS_CASE(K + 6) QS_CASE(K + 7)
+  switch (code) {
+    QS_CASE8(1) QS_CASE8(9) QS_CASE8(17) QS_CASE8(25) QS_CASE8(33) QS_CASE8(41)
+    QS_CASE8(49) QS_CASE8(57) QS_CASE8(65) QS_CASE8(73) QS_CASE8(81) QS_CASE8(89)
+    QS_CASE8(97) QS_CASE8(105)
+    default: break;
+  }
+#undef QS_CASE8
+#undef QS_CASE
+}
+
+// Every op of the sweep on one wave's tile held in registers.
+template <int A, int W, typename R, typename I>
 __device__ __forceinline__ void run_ops(Amp<R, 1 << A> &a, const QS_CONST OpRec *ops,
                                         const QS_CONST R *pool, int nops, const Ctx<I> &c,
                                         const Regs<A, I> &g) {
-#define QS_REG1(BASE, J, CALL) \
-  case (BASE) + (J):           \
-    if constexpr ((J) < A) CALL; \
-    break;
-#define QS_REG2(J0, J1)                                                      \
-  case CODE_DENSE2 + (J0) * kMaxA + (J1):                                    \
-    if constexpr ((J0) < A && (J1) < A && (J0) != (J1))                      \
-      op_dense2<(J0), (J1), A>(a, op, m, c, g);                              \
-    break;
-#define QS_REG2_ROW(J0) QS_REG2(J0, 0) QS_REG2(J0, 1) QS_REG2(J0, 2) QS_REG2(J0, 3) QS_REG2(J0, 4)
-  // one case per lane bit LK = 0..5 at codes BASE + LK
-#define QS_LANE1(BASE, K, CALL) \
-  case (BASE) + (K): {          \
-    constexpr int LK = (K);     \
-    CALL;                       \
-    break;                      \
-  }
-#define QS_LANE6(BASE, CALL)                                                       \
-  QS_LANE1(BASE, 0, CALL) QS_LANE1(BASE, 1, CALL) QS_LANE1(BASE, 2, CALL)          \
-  QS_LANE1(BASE, 3, CALL) QS_LANE1(BASE, 4, CALL) QS_LANE1(BASE, 5, CALL)
-#define QS_MUXRL6(JC)                                                              \
-  QS_LANE6(CODE_MUXRL + (JC) * kLaneBits,                                          \
-           if constexpr ((JC) < A) (op_mux_lane_reg<(JC), LK, A>(a, op, m, c, g)))
-  for (int o = 0; o < nops; ++o) {
-    const QS_CONST OpRec &op = ops[o];
-    const QS_CONST R *m = pool + op.moff;
-    uint32_t code = op.code;
-    // the tile base is the same for every lane of a wave: read it from the
-    // first lane, so that the choices below stay scalar
-    if (op.umask) {
-      const uint32_t tok = __builtin_amdgcn_readfirstlane(
-          (uint32_t)((((uint64_t)c.tbase ^ op.cval) & op.umask) == 0));
-      if (tok)
-        m += op.uoff;
-      else
-        code = op.code_fail;
-    }
-    if (op.bmask)
-      m += 8 * __builtin_amdgcn_readfirstlane(
-                   (uint32_t)(((uint64_t)c.tbase >> op.bshift) & op.bmask));
-    switch (code) {
-      QS_REG1(CODE_U1, 0, (op_1q_reg<0, false, A>(a, op, m, c, g)))
-      QS_REG1(CODE_U1, 1, (op_1q_reg<1, false, A>(a, op, m, c, g)))
-      QS_REG1(CODE_U1, 2, (op_1q_reg<2, false, A>(a, op, m, c, g)))
-      QS_REG1(CODE_U1, 3, (op_1q_reg<3, false, A>(a, op, m, c, g)))
-      QS_REG1(CODE_U1, 4, (op_1q_reg<4, false, A>(a, op, m, c, g)))
-      QS_LANE6(CODE_U1 + kMaxA, (op_1q_lane<false, LK, A>(a, op, m, c, g)))
-      QS_REG1(CODE_C1, 0, (op_1q_reg<0, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_C1, 1, (op_1q_reg<1, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_C1, 2, (op_1q_reg<2, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_C1, 3, (op_1q_reg<3, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_C1, 4, (op_1q_reg<4, true, A>(a, op, m, c, g)))
-      QS_LANE6(CODE_C1 + kMaxA, (op_1q_lane<true, LK, A>(a, op, m, c, g)))
-      QS_REG1(CODE_X1, 0, (op_x1_reg<0, A, R, I>(a, op, c, g)))
-      QS_REG1(CODE_X1, 1, (op_x1_reg<1, A, R, I>(a, op, c, g)))
-      QS_REG1(CODE_X1, 2, (op_x1_reg<2, A, R, I>(a, op, c, g)))
-      QS_REG1(CODE_X1, 3, (op_x1_reg<3, A, R, I>(a, op, c, g)))
-      QS_REG1(CODE_X1, 4, (op_x1_reg<4, A, R, I>(a, op, c, g)))
-      QS_LANE6(CODE_X1 + kMaxA, (op_x1_lane<LK, A, R, I>(a, op, c, g)))
-      case CODE_SPARSE: op_sparse<A>(a, op, m, c, g); break;
-      case CODE_NEG: op_neg<A, R, I>(a, op, c, g); break;
-      case CODE_DIAG: op_diag<A>(a, op, m, c, g); break;
-      QS_REG2_ROW(0)
-      QS_REG2_ROW(1)
-      QS_REG2_ROW(2)
-      QS_REG2_ROW(3)
-      QS_REG2_ROW(4)
-      QS_REG1(CODE_MUXL, 0, (op_mux_reg<0, false, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXL, 1, (op_mux_reg<1, false, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXL, 2, (op_mux_reg<2, false, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXL, 3, (op_mux_reg<3, false, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXL, 4, (op_mux_reg<4, false, A>(a, op, m, c, g)))
-      QS_LANE6(CODE_MUXL + kMaxA, (op_mux_lane<LK, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXR, 0, (op_mux_reg<0, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXR, 1, (op_mux_reg<1, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXR, 2, (op_mux_reg<2, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXR, 3, (op_mux_reg<3, true, A>(a, op, m, c, g)))
-      QS_REG1(CODE_MUXR, 4, (op_mux_reg<4, true, A>(a, op, m, c, g)))
-      QS_MUXRL6(0)
-      QS_MUXRL6(1)
-      QS_MUXRL6(2)
-      QS_MUXRL6(3)
-      QS_MUXRL6(4)
-      case CODE_U1W:
-        if constexpr (W > 0) op_wave<0, A>(a, op, m, c, g);
-        break;
-      case CODE_C1W:
-        if constexpr (W > 0) op_wave<1, A>(a, op, m, c, g);
-        break;
-      case CODE_X1W:
-        if constexpr (W > 0) op_wave<2, A>(a, op, m, c, g);
-        break;
-      case CODE_MUXW:
-        if constexpr (W > 0) op_wave<3, A>(a, op, m, c, g);
-        break;
-      default: break;
-    }
-  }
-#undef QS_MUXRL6
-#undef QS_LANE6
-#undef QS_LANE1
-#undef QS_REG2_ROW
-#undef QS_REG2
-#undef QS_REG1
+  for (int o = 0; o < nops; ++o) run_op<A, W>(a, ops[o], pool, c, g);
 }
 
 // The descriptor's parts, as seen through the constant address space.
@@ -1113,18 +1115,23 @@ __device__ __forceinline__ I wave_offset(const QS_CONST SweepHeader &h, int wave
   return wo;
 }
 
-// One tile per wave (W = 0) or per workgroup (W = kWaveBits): load, apply
-// every op, store.
-//
-// Occupancy: complex128 at A = 4 and complex64 at A = 5 fit in 168 VGPRs,
-// three waves per SIMD (measured: 6.40 ms vs 6.67 ms per 30-qubit complex128
-// sweep at two waves); the few values that do not fit are spilled outside the
-// op loop.  The 64-bit index variants (more than 32 qubits) need more
-// registers and keep the compiler's own choice.
-template <typename R, int A, typename I, int W>
-__global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(((A <= 4 || sizeof(R) == 4) && sizeof(I) == 4) ? 3 : 1))) void k_sweep(
-    typename SV2<R>::T *__restrict__ psi, const char *__restrict__ blob_g, uint32_t xcd_chunk) {
+// Waves per SIMD the sweep kernels are built for: complex128 at A = 4 and
+// complex64 at A = 5 fit in 168 VGPRs, three waves per SIMD (measured: 6.40
+// ms vs 6.67 ms per 30-qubit complex128 sweep at two waves); the few values
+// that do not fit are spilled outside the op loop.  The 64-bit index
+// variants (more than 32 qubits) need more registers and keep the
+// compiler's own choice.
+template <typename R, int A, typename I>
+constexpr int sweep_waves_per_eu() {
+  return ((A <= 4 || sizeof(R) == 4) && sizeof(I) == 4) ? 3 : 1;
+}
+
+// One tile per wave (W = 0) or per workgroup (W = kWaveBits): load, run
+// `ops(a, blob, ctx, regs)` on the tile in registers, store.
+template <typename R, int A, typename I, int W, typename F>
+__device__ __forceinline__ void sweep_tile(typename SV2<R>::T *__restrict__ psi,
+                                           const char *__restrict__ blob_g, uint32_t xcd_chunk,
+                                           F &&ops) {
   constexpr int T = kLaneBits + A + W;
   constexpr int NA = 1 << A;
   const Blob<R> b = open_blob<R>(blob_g);
@@ -1152,7 +1159,7 @@ __attribute__((amdgpu_waves_per_eu(((A <= 4 || sizeof(R) == 4) && sizeof(I) == 4
       a.im[r] = v[r].y;
     }
   }
-  run_ops<A, W>(a, b.ops, b.pool, (int)h.nops, c, g);
+  ops(a, b, c, g);
   typename SNT<R>::T *dst = reinterpret_cast<typename SNT<R>::T *>(psi);
   // the store addresses are recomputed from an opaque copy of the tile base:
   // keeping the 2^A load addresses alive across the op loop costs 2^(A+1)
@@ -1166,6 +1173,22 @@ __attribute__((amdgpu_waves_per_eu(((A <= 4 || sizeof(R) == 4) && sizeof(I) == 4
     v.y = a.im[r];
     __builtin_nontemporal_store(v, dst + (sbase | goff<A>(g, r)));
   }
+}
+
+// The generic sweep kernel: the op list is data, one switch per op.
+template <typename R, int A, typename I, int W>
+struct SweepOps {
+  __device__ __forceinline__ void operator()(Amp<R, 1 << A> &a, const Blob<R> &b,
+                                             const Ctx<I> &c, const Regs<A, I> &g) const {
+    run_ops<A, W>(a, b.ops, b.pool, (int)b.h->nops, c, g);
+  }
+};
+
+template <typename R, int A, typename I, int W>
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<R, A, I>()))) void k_sweep(
+    typename SV2<R>::T *__restrict__ psi, const char *__restrict__ blob_g, uint32_t xcd_chunk) {
+  sweep_tile<R, A, I, W>(psi, blob_g, xcd_chunk, SweepOps<R, A, I, W>());
 }
 
 }  // namespace sweep

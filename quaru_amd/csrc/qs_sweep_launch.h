@@ -23,6 +23,15 @@ template <typename R, int A>
 hipError_t launch_sweep_ra(int W, int n, hipStream_t stream, void *psi, const void *d_blob,
                            uint64_t ntiles, bool xcd);
 
+// Grid of a sweep over `ntiles` tiles: one workgroup per tile (W = 2) or
+// per four wave tiles (W = 0); with `xcd`, each XCD walks one contiguous
+// eighth of the tiles when the grid divides by 8 (xcd_chunk = grid / 8).
+inline void sweep_grid(int W, uint64_t ntiles, bool xcd, uint64_t *grid, uint32_t *xcd_chunk) {
+  const uint64_t per_block = W == 0 ? 4 : 1;
+  *grid = (ntiles + per_block - 1) / per_block;
+  *xcd_chunk = (xcd && *grid % 8 == 0) ? (uint32_t)(*grid / 8) : 0u;
+}
+
 #define QS_SWEEP_LAUNCH_DECL(R, A)                                                       \
   template <>                                                                            \
   hipError_t launch_sweep_ra<R, A>(int W, int n, hipStream_t stream, void *psi,          \

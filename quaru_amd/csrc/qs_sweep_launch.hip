@@ -19,9 +19,9 @@ template <typename R, int A, int W>
 hipError_t launch(int n, hipStream_t stream, void *psi, const void *d_blob, uint64_t ntiles,
                   bool xcd) {
   using V = typename SV2<R>::T;
-  const uint64_t per_block = W == 0 ? kThreads / kWave : 1;
-  const uint64_t grid = (ntiles + per_block - 1) / per_block;
-  const uint32_t xcd_chunk = (xcd && grid % 8 == 0) ? (uint32_t)(grid / 8) : 0u;
+  uint64_t grid;
+  uint32_t xcd_chunk;
+  sweep_grid(W, ntiles, xcd, &grid, &xcd_chunk);
   if (n <= 32)
     hipLaunchKernelGGL((k_sweep<R, A, uint32_t, W>), dim3((unsigned)grid), dim3(kThreads), 0,
                        stream, (V *)psi, (const char *)d_blob, xcd_chunk);

@@ -10,6 +10,12 @@ if ROOT not in sys.path:
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# Sweep kernels specialised at run time (hipRTC) compile on background
+# threads by default; the suite runs the generic sweep kernel everywhere
+# (deterministic, no compile threads beside the CPU oracle) and checks the
+# specialised kernels in tests of their own, which switch them on.
+os.environ.setdefault("QS_SWEEP_JIT", "0")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
