@@ -27,7 +27,8 @@ Reported on one JSON line:
                32 x 32 blocks vs one gate per qubit, complex128 and complex64
 
 ``--profile SECTION`` runs a 1 GiB copy (counter calibration) and then one
-step of that section only, for rocprofv3.
+step of that section only, for rocprofv3 (c128 and c64: after one warmup
+step, so the profiled step runs the specialised sweep kernels).
 """
 from __future__ import annotations
 
@@ -101,17 +102,23 @@ def run_steps(reg: Register, opl, steps: int) -> float:
     return time.perf_counter() - t0
 
 
+def jit_stats():
+    """(compiled, failed, launches) of the specialised sweep kernels so far."""
+    import ctypes
+    lib = _lib.load()
+    c, f, n = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    lib.qs_sweep_jit_stats(ctypes.byref(c), ctypes.byref(f), ctypes.byref(n))
+    return c.value, f.value, n.value
+
+
 def jit_wait():
     """Wait for the sweep kernels specialised during the warmup (compiled on
     host threads while the warmup ran the generic kernel) so that the timed
     steps run them; logs the totals."""
-    import ctypes
-    lib = _lib.load()
     t0 = time.perf_counter()
-    lib.qs_sweep_jit_wait()
-    c, f, n = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-    lib.qs_sweep_jit_stats(ctypes.byref(c), ctypes.byref(f), ctypes.byref(n))
-    log(f"specialised sweep kernels: {c.value} compiled, {f.value} failed "
+    _lib.load().qs_sweep_jit_wait()
+    c, f, _ = jit_stats()
+    log(f"specialised sweep kernels: {c} compiled, {f} failed "
         f"(waited {time.perf_counter() - t0:.1f} s)")
 
 
@@ -128,10 +135,13 @@ def timed_run(reg: Register, opl, steps: int, warmup: int):
     reg.stats(reset=True)
     timing(reg)
     timing_enable(reg, True)
+    jit0 = jit_stats()[2]
     dt = run_steps(reg, opl, steps)
     timing_enable(reg, False)
     ms, cnt = timing(reg)
-    return dt, ms, cnt, reg.stats(reset=True)
+    st = reg.stats(reset=True)
+    st["specialised_launches"] = jit_stats()[2] - jit0
+    return dt, ms, cnt, st
 
 
 def cpu_baseline(w: circuits.Workload, threads: int):
@@ -442,7 +452,8 @@ def main() -> int:
                              "grover_unfused", "grover_c64", "grover_c64_unfused",
                              "block5_c64", "block5_c128", "sweep_ops"],
                     help="short run for rocprofv3: a 1 GiB torch copy (counter "
-                         "calibration), then one warmup-free step of one section")
+                         "calibration), then one step of one section (c128, c64: "
+                         "after one warmup step)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the state-vector path is built and measured on a "
@@ -454,7 +465,9 @@ def main() -> int:
                 "cpu": not args.no_cpu, "grover32": not args.no_grover32,
                 "block5": not args.no_block5}
     if args.profile:
-        args.steps, args.warmup = 1, 0
+        # c128 / c64: one warmup step first, whose sweeps queue the compiles of
+        # the specialised sweep kernels that the profiled step then runs
+        args.steps, args.warmup = 1, (1 if args.profile in ("c128", "c64") else 0)
         sections = {k: False for k in sections}
         if args.profile.startswith("grover"):
             sections["grover32"] = True
@@ -510,7 +523,9 @@ def main() -> int:
             "unit": "GB/s", "frac": achieved / HBM_PEAK,
             "traffic": (recorded_traffic("k_sweep_c128_30q.md") if n == 30 else None),
             "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/k_sweep_c128_30q.md",
-            "kernel": "k_sweep (fused sweep, complex128)", "launches": cnt[0],
+            "kernel": "qs_sweep_jit / k_sweep (fused sweep, complex128; kernels specialised "
+                      "to each sweep's op-code sequence)", "launches": cnt[0],
+            "specialised_launches": st["specialised_launches"],
             "avg_ms": sweep_avg_s * 1e3, "bytes_per_launch": sweep_bytes}
         out["sweeps_per_step"] = st["sweeps"] / args.steps
         out["gates_per_sweep"] = G * args.steps / max(st["sweeps"], 1)
@@ -530,7 +545,9 @@ def main() -> int:
         ach64 = 2.0 * (1 << n) * 8 / avg64 if cnt64[0] else 0.0
         g64 = G * args.steps / dt64
         entry = {"gates_per_s": g64,
-                 "roofline": {"kernel": "k_sweep (fused sweep, complex64)",
+                 "roofline": {"kernel": "qs_sweep_jit / k_sweep (fused sweep, complex64; "
+                                        "specialised kernels)",
+                              "specialised_launches": st64["specialised_launches"],
                               "achieved": ach64 / 1e9, "unit": "GB/s",
                               "frac": ach64 / HBM_PEAK, "avg_ms": avg64 * 1e3,
                               "launches": cnt64[0],

@@ -134,12 +134,16 @@ def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> st
 CPP_DIR = os.path.join(HERE, "cpp")
 BIN_DIR = os.path.join(HERE, "bin")
 PROGRAMS = ["register_operation_tests", "examples", "adjoint_ansatz"]
+# HIP programs built from the kernel headers alone (no library): sweep_memory
+# times the fused sweep's load/store path for given tile shapes.
+HIP_PROGRAMS = ["sweep_memory"]
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
 def build_programs(verbose: bool = False) -> list:
-    """g++ each program in ``quaru_amd/cpp`` into ``quaru_amd/bin``; the
-    binaries find the library through an rpath relative to themselves."""
+    """g++ each C++ program in ``quaru_amd/cpp`` into ``quaru_amd/bin`` (the
+    binaries find the library through an rpath relative to themselves), and
+    hipcc each HIP program there."""
     os.makedirs(BIN_DIR, exist_ok=True)
     cxx = os.environ.get("CXX", "g++")
     procs = []
@@ -152,6 +156,14 @@ def build_programs(verbose: bool = False) -> list:
             print(" ".join(cmd), flush=True)
         procs.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE,
                                              stderr=subprocess.STDOUT, text=True)))
+    for name in HIP_PROGRAMS:
+        cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC,
+               os.path.join(CPP_DIR, name + ".hip"), "-o", os.path.join(BIN_DIR, name)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                             stderr=subprocess.STDOUT, text=True)))
     failed = []
     for name, p in procs:
         log, _ = p.communicate()
@@ -159,7 +171,7 @@ def build_programs(verbose: bool = False) -> list:
             failed.append((name, log))
     if failed:
         raise RuntimeError("g++ failed:\n" + "\n".join(f"--- {n}\n{o}" for n, o in failed))
-    return [os.path.join(BIN_DIR, n) for n in PROGRAMS]
+    return [os.path.join(BIN_DIR, n) for n in PROGRAMS + HIP_PROGRAMS]
 
 
 if __name__ == "__main__":

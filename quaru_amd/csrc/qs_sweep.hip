@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -617,6 +618,18 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     }
     h.nops = (uint32_t)ops.size();
     if (run.sweep_ops) *run.sweep_ops += ops.size();
+    // QS_SWEEP_TRACE=1: one line per planned sweep on stderr, the tile's
+    // roles (free lane bits, register slots, wave bits) and its op count
+    static const bool trace = env_int("QS_SWEEP_TRACE", 0) != 0;
+    if (trace) {
+      std::fprintf(stderr, "qs_sweep n %d lanes", n);
+      for (int i = 0; i < nfree; ++i) std::fprintf(stderr, " %d", lanes[i]);
+      std::fprintf(stderr, " regs");
+      for (int j = 0; j < A; ++j) std::fprintf(stderr, " %d", regs[j]);
+      std::fprintf(stderr, " waves");
+      for (int k = 0; k < W; ++k) std::fprintf(stderr, " %d", waves[k]);
+      std::fprintf(stderr, " ops %u\n", h.nops);
+    }
     h.npool = (uint32_t)pool.size();
     h.ntiles = 1ull << (n - T);
     const size_t hbytes = (sizeof(SweepHeader) + 15) & ~(size_t)15;
