@@ -280,9 +280,10 @@ int qs_adjoint_stats(const qs_adjoint *ctx, uint64_t *sweeps, uint64_t *kernels)
 
 /* Sweep kernels specialised to one op-code sequence, compiled at run time
  * with hipRTC (no reference counterpart; engine tuning).  mode 0: off; 1
- * (default, or QS_SWEEP_JIT): a new sequence runs the generic kernel while
- * its specialised kernel compiles on host threads; 2: compile before the
- * first launch.  mode < 0 only queries.  Returns the previous mode. */
+ * (default, or QS_SWEEP_JIT): for states of 22 qubits and more, a new
+ * sequence runs the generic kernel while its specialised kernel compiles on
+ * host threads; 2: every sequence, compiled before its first launch.
+ * mode < 0 only queries.  Returns the previous mode. */
 int qs_sweep_jit_mode(int mode);
 
 /* Blocks until every queued compile has finished. */

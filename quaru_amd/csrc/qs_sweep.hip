@@ -289,8 +289,10 @@ hipError_t launch_sweep_a(int A, int W, SweepRunner &run, const void *d_blob, ui
                           const std::vector<OpRec> &ops) {
   const bool xcd = sweep_xcd_chunks();
   // a kernel specialised to this op-code sequence, when one is compiled
-  // (32-bit indices only)
-  if (run.n <= 32 && jit_mode() != 0) {
+  // (32-bit indices only; in the automatic mode only from 22 qubits, where a
+  // pass over the state takes long enough to repay a compile)
+  const int jm = run.n <= 32 ? jit_mode() : 0;
+  if (jm == 2 || (jm == 1 && run.n >= 22)) {
     const bool c64 = std::is_same<R, float>::value;
     hipFunction_t fn = jit_kernel(jit_signature(ops, c64, A, W), ops, c64, A, W);
     if (fn) {

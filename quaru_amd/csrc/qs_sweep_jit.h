@@ -10,11 +10,11 @@
 // every sweep with the same code sequence (a repeated circuit, the layers of
 // a circuit, the steps of an optimiser).
 //
-// Modes (QS_SWEEP_JIT, or qs_sweep_jit_mode): 0 off; 1 (default) a new code
-// sequence launches the generic kernel and queues a compile on a pool of
-// host threads, later sweeps with that sequence use the compiled kernel;
-// 2 compile before the first launch.  A failed compile leaves the sequence
-// on the generic kernel.
+// Modes (QS_SWEEP_JIT, or qs_sweep_jit_mode): 0 off; 1 (default) for states
+// of 22 qubits and more, a new code sequence launches the generic kernel and
+// queues a compile on two host threads, later sweeps with that sequence use
+// the compiled kernel; 2 every sweep, compiled before its first launch.  A
+// failed compile leaves the sequence on the generic kernel.
 #pragma once
 
 #include <hip/hip_runtime.h>

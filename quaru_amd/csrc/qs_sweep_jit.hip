@@ -4,14 +4,15 @@
 // The kernel source is the sweep kernel header itself (embedded at build
 // time) plus a generated body: the op functions of the sweep's code
 // sequence, called in order with their codes as template arguments.  The
-// compile runs on a pool of host threads with hipRTC for gfx950, with the
-// same code-generation flags as the library (no packed FP32).
+// compile runs on two host threads with hipRTC for gfx950, with the same
+// code-generation flags as the library (no packed FP32).
 #include "qs_sweep_jit.h"
 
 #include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +21,8 @@
 #include <mutex>
 #include <sstream>
 #include <thread>
+
+#include <unistd.h>
 
 #include "../../include/quaru_hip.h"
 #include "qs_sweep_kernel.h"
@@ -125,25 +128,60 @@ void worker() {
   }
 }
 
-// At exit: no new compiles; the running ones finish before the runtime's
-// own exit handlers run (registered after hipRTC's, so called before them).
-void shutdown() {
+// At exit (an on_exit handler, so the exit status is known): no new
+// compiles, queued ones are dropped, and the running ones (at most one per
+// compile thread, about 0.5 s each) get 5 s (QS_SWEEP_JIT_EXIT_WAIT) to
+// finish before hipRTC's own
+// exit-time teardown, which start_workers orders after this handler.  A
+// compile still inside hipRTC after that would be torn down under it (a
+// crash or a hang), so the rest of the exit is skipped instead: stdio is
+// flushed and the process ends at once with its exit status.
+void shutdown(int status, void *) {
   Jit &j = jit();
   std::unique_lock<std::mutex> lk(j.mu);
   j.stopping = true;
   j.busy -= (int)j.queue.size();
   j.queue.clear();
   j.work.notify_all();
-  j.done.wait(lk, [&] { return j.busy <= 0; });
+  static const char *w = std::getenv("QS_SWEEP_JIT_EXIT_WAIT");  // seconds
+  const int wait_ms = w ? std::max(0, (int)(std::atof(w) * 1000.0)) : 5000;
+  if (!j.done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return j.busy <= 0; })) {
+    std::fflush(nullptr);
+    _exit(status);
+  }
 }
+
+std::string source(const std::vector<OpRec> &ops, bool c64, int A, int W);
+
+// hipRTC and the compiler library behind it register parts of their
+// exit-time teardown on first use, and exit handlers run in the reverse
+// order of registration.  A handler registered before that first use runs
+// after the teardown, while compile threads may still be inside
+// hiprtcCompileProgram: the process then crashes or hangs at exit.  One
+// compile of a sweep kernel on this thread before `shutdown` is registered
+// puts that teardown first, so `shutdown` runs (and waits for the compile
+// threads) before it.  The kernel is discarded.
+void warm_up() {
+  std::vector<OpRec> ops(3);
+  ops[0].code = CODE_U1;
+  ops[1].code = CODE_C1 + kMaxA;  // a lane-bit form
+  ops[2].code = CODE_NEG;
+  std::vector<char> code;
+  std::string log;
+  (void)compile(source(ops, false, 4, kWaveBits), code, log);
+}
+
+// hipRTC runs one compile at a time (8 specialised kernels on 8 threads:
+// 5.1 s wall for 5.3 s of CPU), so two threads keep it busy; at exit at
+// most two compiles are still running.
+constexpr int kCompileThreads = 2;
 
 void start_workers(Jit &j) {
   if (j.started) return;
   j.started = true;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const int n = (int)std::min(16u, hw);
-  for (int i = 0; i < n; ++i) std::thread(worker).detach();
-  std::atexit(shutdown);
+  warm_up();
+  on_exit(shutdown, nullptr);
+  for (int i = 0; i < kCompileThreads; ++i) std::thread(worker).detach();
 }
 
 std::string source(const std::vector<OpRec> &ops, bool c64, int A, int W) {

@@ -1,0 +1,64 @@
+"""Process exit while sweep kernels compile at run time (hipRTC, no GPU
+needed): the library's exit handler waits for the running compiles (or,
+after its limit, ends the process with its exit status), and hipRTC's own
+exit-time teardown must come after it.  A handler registered
+before hipRTC's first use ran after that teardown, and a process that
+exited with compiles in flight crashed or hung.  The compiler cache is
+switched off so that every compile really runs."""
+import os
+import subprocess
+import time
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "quaru_amd", "csrc")
+LIBDIR = os.path.join(ROOT, "quaru_amd")
+SRC = os.path.join(ROOT, "tests", "cpp", "jit_exit.cpp")
+HIPCC = "/opt/rocm/bin/hipcc"
+
+pytestmark = pytest.mark.skipif(
+    not (os.path.exists(HIPCC) and os.path.exists(os.path.join(LIBDIR, "libquaru_hip.so"))),
+    reason="needs hipcc and the built library")
+
+
+@pytest.fixture(scope="module")
+def prog(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("jit_exit") / "jit_exit")
+    subprocess.run([HIPCC, "-std=c++17", "-O1", "-I", CSRC, SRC, "-L", LIBDIR, "-lquaru_hip",
+                    f"-Wl,-rpath,{LIBDIR}", "-o", out],
+                   check=True, capture_output=True, timeout=600)
+    return out
+
+
+def _run(prog, *args, **env_extra):
+    env = dict(os.environ, AMD_COMGR_CACHE="0", QS_SWEEP_JIT="1", **env_extra)
+    t0 = time.perf_counter()
+    p = subprocess.run([prog, *args], env=env, capture_output=True, text=True, timeout=120)
+    return p, time.perf_counter() - t0
+
+
+def test_compiles_succeed(prog):
+    p, _ = _run(prog, "wait")
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "compiled 8 failed 0" in p.stdout, p.stdout + p.stderr
+
+
+def test_exit_with_compiles_in_flight(prog):
+    for _ in range(6):
+        p, dt = _run(prog)
+        assert p.returncode == 0, f"exit status {p.returncode}\n{p.stdout}{p.stderr}"
+        assert "main returns" in p.stdout
+        # at most the two running compiles finish after main returns (the
+        # exit handler waits for them 5 s at most)
+        assert dt < 30.0, f"exit took {dt:.1f} s"
+
+
+def test_exit_status_when_compiles_are_cut_short(prog):
+    """With no time to wait (QS_SWEEP_JIT_EXIT_WAIT=0) a compile still running
+    at exit ends the process at once: stdout flushed, exit status kept."""
+    for status in ("0", "3"):
+        p, dt = _run(prog, "status", status, QS_SWEEP_JIT_EXIT_WAIT="0")
+        assert p.returncode == int(status), f"exit status {p.returncode}\n{p.stdout}{p.stderr}"
+        assert "main returns" in p.stdout
+        assert dt < 30.0
