@@ -281,8 +281,9 @@ int qs_adjoint_stats(const qs_adjoint *ctx, uint64_t *sweeps, uint64_t *kernels)
 /* Sweep kernels specialised to one op-code sequence, compiled at run time
  * with hipRTC (no reference counterpart; engine tuning).  mode 0: off; 1
  * (default, or QS_SWEEP_JIT): for states of 22 qubits and more, a new
- * sequence runs the generic kernel while its specialised kernel compiles on
- * host threads; 2: every sequence, compiled before its first launch.
+ * sequence runs the generic kernel while its specialised kernel compiles in
+ * a helper process (bin/qs_sweep_compile beside the library, or
+ * QS_SWEEP_COMPILER); 2: every sequence, compiled before its first launch.
  * mode < 0 only queries.  Returns the previous mode. */
 int qs_sweep_jit_mode(int mode);
 

@@ -117,7 +117,7 @@ def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> st
             msg = "\n".join(f"--- {s}\n{o}" for s, o in failed)
             raise RuntimeError(f"hipcc failed:\n{msg}")
         tmp = LIB + ".tmp"
-        cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lhiprtc", "-o", tmp]
+        cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
@@ -134,9 +134,11 @@ def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> st
 CPP_DIR = os.path.join(HERE, "cpp")
 BIN_DIR = os.path.join(HERE, "bin")
 PROGRAMS = ["register_operation_tests", "examples", "adjoint_ansatz"]
-# HIP programs built from the kernel headers alone (no library): sweep_memory
-# times the fused sweep's load/store path for given tile shapes.
-HIP_PROGRAMS = ["sweep_memory"]
+# Programs built with hipcc (no library): sweep_memory times the fused
+# sweep's load/store path for given tile shapes; qs_sweep_compile is the
+# helper process that compiles specialised sweep kernels with hipRTC.
+HIP_PROGRAMS = {"sweep_memory": ("sweep_memory.hip", []),
+                "qs_sweep_compile": ("qs_sweep_compile.cpp", ["-lhiprtc"])}
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
@@ -156,10 +158,10 @@ def build_programs(verbose: bool = False) -> list:
             print(" ".join(cmd), flush=True)
         procs.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE,
                                              stderr=subprocess.STDOUT, text=True)))
-    for name in HIP_PROGRAMS:
+    for name, (src, libs) in HIP_PROGRAMS.items():
         cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC,
-               os.path.join(CPP_DIR, name + ".hip"), "-o", os.path.join(BIN_DIR, name)]
+               os.path.join(CPP_DIR, src), *libs, "-o", os.path.join(BIN_DIR, name)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE,
@@ -171,7 +173,7 @@ def build_programs(verbose: bool = False) -> list:
             failed.append((name, log))
     if failed:
         raise RuntimeError("g++ failed:\n" + "\n".join(f"--- {n}\n{o}" for n, o in failed))
-    return [os.path.join(BIN_DIR, n) for n in PROGRAMS + HIP_PROGRAMS]
+    return [os.path.join(BIN_DIR, n) for n in PROGRAMS + list(HIP_PROGRAMS)]
 
 
 if __name__ == "__main__":

@@ -12,9 +12,11 @@
 //
 // Modes (QS_SWEEP_JIT, or qs_sweep_jit_mode): 0 off; 1 (default) for states
 // of 22 qubits and more, a new code sequence launches the generic kernel and
-// queues a compile on two host threads, later sweeps with that sequence use
-// the compiled kernel; 2 every sweep, compiled before its first launch.  A
-// failed compile leaves the sequence on the generic kernel.
+// queues a compile (host threads start one helper process per kernel,
+// bin/qs_sweep_compile beside the library), later sweeps with that sequence
+// use the compiled kernel; 2 every sweep, compiled before its first launch.
+// A failed compile leaves the sequence on the generic kernel.  At exit the
+// helpers still running are stopped; nothing waits for them.
 #pragma once
 
 #include <hip/hip_runtime.h>

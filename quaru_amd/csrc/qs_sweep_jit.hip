@@ -3,29 +3,39 @@
 //
 // The kernel source is the sweep kernel header itself (embedded at build
 // time) plus a generated body: the op functions of the sweep's code
-// sequence, called in order with their codes as template arguments.  The
-// compile runs on two host threads with hipRTC for gfx950, with the same
-// code-generation flags as the library (no packed FP32).
+// sequence, called in order with their codes as template arguments.  Each
+// kernel is compiled with hipRTC for gfx950, with the library's
+// code-generation flags (no packed FP32), by a helper process
+// (bin/qs_sweep_compile beside the library) that host threads start.
 #include "qs_sweep_jit.h"
 
-#include <hip/hiprtc.h>
+#include <dlfcn.h>
+#include <fcntl.h>
+#include <poll.h>
+#include <signal.h>
+#include <spawn.h>
+#include <sys/wait.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cerrno>
 #include <cstdlib>
+#include <cstring>
 #include <deque>
 #include <map>
+#include <vector>
 #include <mutex>
 #include <sstream>
 #include <thread>
 
-#include <unistd.h>
-
 #include "../../include/quaru_hip.h"
 #include "qs_sweep_kernel.h"
+
+extern char **environ;
 
 namespace qs {
 namespace sweep {
@@ -34,6 +44,10 @@ namespace {
 const char kKernelHeader[] =
 #include "qs_sweep_kernel_src.inc"
     ;
+
+// hipRTC options: the library's code-generation flags (no packed FP32).
+const char *const kCompileOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
+                                       "-target-feature",       "-Xclang", "-packed-fp32-ops"};
 
 // Specialised kernels kept per process (a long run over ever-new circuits
 // stops compiling there and stays on the generic kernel).
@@ -53,6 +67,7 @@ struct Jit {
   std::map<std::string, Entry> table;
   std::deque<std::pair<std::string, std::string>> queue;  // signature, source
   int busy = 0;  // queued + compiling
+  std::vector<pid_t> children;  // helpers started and not yet reaped
   bool started = false, stopping = false;
   int mode = 1;
   std::atomic<uint64_t> compiled{0}, failed{0}, launches{0};
@@ -68,42 +83,150 @@ Jit &jit() {
   return *j;
 }
 
+// The compiler helper: bin/qs_sweep_compile beside the library (or
+// QS_SWEEP_COMPILER).  Empty if it cannot be found.
+const std::string &helper_path() {
+  static const std::string path = [] {
+    const char *e = std::getenv("QS_SWEEP_COMPILER");
+    if (e && *e) return std::string(e);
+    Dl_info info;
+    if (!dladdr((void *)&helper_path, &info) || !info.dli_fname) return std::string();
+    std::string lib = info.dli_fname;
+    const size_t slash = lib.rfind('/');
+    const std::string dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
+    const std::string p = dir + "/bin/qs_sweep_compile";
+    return access(p.c_str(), X_OK) == 0 ? p : std::string();
+  }();
+  return path;
+}
+
+void forget_child(pid_t pid);
+void track_child(pid_t pid);
+
+// Compile `src` in a child process running the helper: the header and the
+// source go in through a pipe, the code object comes back through another,
+// and the compiler log through a third.  hipRTC itself never loads into
+// this process.
 bool compile(const std::string &src, std::vector<char> &code, std::string &log) {
-  hiprtcProgram prog;
-  const char *headers[] = {kKernelHeader};
-  const char *names[] = {"qs_sweep_kernel.h"};
-  if (hiprtcCreateProgram(&prog, src.c_str(), "qs_sweep_jit.hip", 1, headers, names) !=
-      HIPRTC_SUCCESS)
+  const std::string &helper = helper_path();
+  if (helper.empty()) {
+    log = "compiler helper bin/qs_sweep_compile not found beside the library";
     return false;
-  const char *opts[] = {"--offload-arch=gfx950", "-O3",           "-std=c++17", "-Xclang",
-                        "-target-feature",       "-Xclang",       "-packed-fp32-ops"};
-  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
-  size_t n = 0;
-  if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
-    log.resize(n);
-    hiprtcGetProgramLog(prog, &log[0]);
   }
-  bool ok = rc == HIPRTC_SUCCESS;
-  if (ok) {
-    size_t sz = 0;
-    ok = hiprtcGetCodeSize(prog, &sz) == HIPRTC_SUCCESS && sz > 0;
-    if (ok) {
-      code.resize(sz);
-      ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+  int in[2], out[2], err[2];
+  if (pipe2(in, O_CLOEXEC)) return false;
+  if (pipe2(out, O_CLOEXEC)) {
+    close(in[0]);
+    close(in[1]);
+    return false;
+  }
+  if (pipe2(err, O_CLOEXEC)) {
+    close(in[0]);
+    close(in[1]);
+    close(out[0]);
+    close(out[1]);
+    return false;
+  }
+  posix_spawn_file_actions_t fa;
+  posix_spawn_file_actions_init(&fa);
+  posix_spawn_file_actions_adddup2(&fa, in[0], 0);
+  posix_spawn_file_actions_adddup2(&fa, out[1], 1);
+  posix_spawn_file_actions_adddup2(&fa, err[1], 2);
+  posix_spawnattr_t at;
+  posix_spawnattr_init(&at);
+  sigset_t none;
+  sigemptyset(&none);
+  posix_spawnattr_setsigmask(&at, &none);
+  posix_spawnattr_setflags(&at, POSIX_SPAWN_SETSIGMASK);
+  std::vector<char *> argv;
+  argv.push_back(const_cast<char *>(helper.c_str()));
+  for (const char *o : kCompileOptions) argv.push_back(const_cast<char *>(o));
+  argv.push_back(nullptr);
+  pid_t pid = 0;
+  int rc;
+  {
+    // registered under the lock, so the exit handler either sees the child
+    // or runs before it exists
+    std::lock_guard<std::mutex> lk(jit().mu);
+    rc = posix_spawn(&pid, helper.c_str(), &fa, &at, argv.data(), environ);
+    if (rc == 0) track_child(pid);
+  }
+  posix_spawn_file_actions_destroy(&fa);
+  posix_spawnattr_destroy(&at);
+  close(in[0]);
+  close(out[1]);
+  close(err[1]);
+  if (rc != 0) {
+    close(in[1]);
+    close(out[0]);
+    close(err[0]);
+    log = "cannot start " + helper;
+    return false;
+  }
+  // input: header length, header, source (the helper reads all of it before
+  // it writes anything)
+  std::string msg(8, '\0');
+  const uint64_t hl = sizeof(kKernelHeader) - 1;
+  std::memcpy(&msg[0], &hl, 8);
+  msg.append(kKernelHeader, hl);
+  msg += src;
+  bool wrote = true;
+  for (size_t off = 0; off < msg.size();) {
+    const ssize_t w = write(in[1], msg.data() + off, msg.size() - off);
+    if (w < 0 && errno == EINTR) continue;
+    if (w <= 0) {
+      wrote = false;
+      break;
+    }
+    off += (size_t)w;
+  }
+  close(in[1]);
+  // output and log, read together so that neither pipe fills up
+  pollfd fds[2] = {{out[0], POLLIN, 0}, {err[0], POLLIN, 0}};
+  int open_fds = 2;
+  char buf[1 << 16];
+  while (open_fds > 0) {
+    if (poll(fds, 2, -1) < 0) {
+      if (errno == EINTR) continue;
+      break;
+    }
+    for (int k = 0; k < 2; ++k) {
+      if (fds[k].fd < 0 || !(fds[k].revents & (POLLIN | POLLHUP | POLLERR))) continue;
+      const ssize_t r = read(fds[k].fd, buf, sizeof buf);
+      if (r < 0 && errno == EINTR) continue;
+      if (r <= 0) {
+        close(fds[k].fd);
+        fds[k].fd = -1;
+        --open_fds;
+      } else if (k == 0) {
+        code.insert(code.end(), buf, buf + r);
+      } else if (log.size() < 16384) {
+        log.append(buf, (size_t)r);
+      }
     }
   }
-  hiprtcDestroyProgram(&prog);
-  return ok;
+  for (int k = 0; k < 2; ++k)
+    if (fds[k].fd >= 0) close(fds[k].fd);
+  forget_child(pid);  // before the wait: its pid stays ours until reaped
+  int status = 0;
+  while (waitpid(pid, &status, 0) < 0 && errno == EINTR) {
+  }
+  return wrote && WIFEXITED(status) && WEXITSTATUS(status) == 0 && !code.empty();
 }
 
 void worker() {
+  // a helper that dies early must not stop the process with SIGPIPE
+  sigset_t pipe_set;
+  sigemptyset(&pipe_set);
+  sigaddset(&pipe_set, SIGPIPE);
+  pthread_sigmask(SIG_BLOCK, &pipe_set, nullptr);
   Jit &j = jit();
   for (;;) {
     std::pair<std::string, std::string> item;
     {
       std::unique_lock<std::mutex> lk(j.mu);
       j.work.wait(lk, [&] { return !j.queue.empty() || j.stopping; });
-      if (j.stopping && j.queue.empty()) return;
+      if (j.stopping) return;
       item = std::move(j.queue.front());
       j.queue.pop_front();
     }
@@ -112,6 +235,7 @@ void worker() {
     const bool ok = compile(item.second, code, log);
     {
       std::lock_guard<std::mutex> lk(j.mu);
+      if (j.stopping) return;
       Entry &e = j.table[item.first];
       e.state = ok ? 1 : 2;
       e.code.swap(code);
@@ -128,60 +252,38 @@ void worker() {
   }
 }
 
-// At exit (an on_exit handler, so the exit status is known): no new
-// compiles, queued ones are dropped, and the running ones (at most one per
-// compile thread, about 0.5 s each) get 5 s (QS_SWEEP_JIT_EXIT_WAIT) to
-// finish before hipRTC's own
-// exit-time teardown, which start_workers orders after this handler.  A
-// compile still inside hipRTC after that would be torn down under it (a
-// crash or a hang), so the rest of the exit is skipped instead: stdio is
-// flushed and the process ends at once with its exit status.
-void shutdown(int status, void *) {
+void track_child(pid_t pid) { jit().children.push_back(pid); }
+
+void forget_child(pid_t pid) {
   Jit &j = jit();
-  std::unique_lock<std::mutex> lk(j.mu);
+  std::lock_guard<std::mutex> lk(j.mu);
+  j.children.erase(std::remove(j.children.begin(), j.children.end(), pid), j.children.end());
+}
+
+// At exit: no new compiles, and the running ones are stopped (SIGKILL to
+// the helpers this library started and has not yet reaped, so none of
+// those pids can belong to another process).  Nothing waits: the compile
+// threads find their helper gone and return.
+void shutdown() {
+  Jit &j = jit();
+  std::lock_guard<std::mutex> lk(j.mu);
   j.stopping = true;
-  j.busy -= (int)j.queue.size();
   j.queue.clear();
+  for (pid_t pid : j.children) kill(pid, SIGKILL);
   j.work.notify_all();
-  static const char *w = std::getenv("QS_SWEEP_JIT_EXIT_WAIT");  // seconds
-  const int wait_ms = w ? std::max(0, (int)(std::atof(w) * 1000.0)) : 5000;
-  if (!j.done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return j.busy <= 0; })) {
-    std::fflush(nullptr);
-    _exit(status);
-  }
 }
 
-std::string source(const std::vector<OpRec> &ops, bool c64, int A, int W);
-
-// hipRTC and the compiler library behind it register parts of their
-// exit-time teardown on first use, and exit handlers run in the reverse
-// order of registration.  A handler registered before that first use runs
-// after the teardown, while compile threads may still be inside
-// hiprtcCompileProgram: the process then crashes or hangs at exit.  One
-// compile of a sweep kernel on this thread before `shutdown` is registered
-// puts that teardown first, so `shutdown` runs (and waits for the compile
-// threads) before it.  The kernel is discarded.
-void warm_up() {
-  std::vector<OpRec> ops(3);
-  ops[0].code = CODE_U1;
-  ops[1].code = CODE_C1 + kMaxA;  // a lane-bit form
-  ops[2].code = CODE_NEG;
-  std::vector<char> code;
-  std::string log;
-  (void)compile(source(ops, false, 4, kWaveBits), code, log);
+// Compiles run in child processes, so they run in parallel.
+int compile_threads() {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::min(8u, hw);
 }
-
-// hipRTC runs one compile at a time (8 specialised kernels on 8 threads:
-// 5.1 s wall for 5.3 s of CPU), so two threads keep it busy; at exit at
-// most two compiles are still running.
-constexpr int kCompileThreads = 2;
 
 void start_workers(Jit &j) {
   if (j.started) return;
   j.started = true;
-  warm_up();
-  on_exit(shutdown, nullptr);
-  for (int i = 0; i < kCompileThreads; ++i) std::thread(worker).detach();
+  std::atexit(shutdown);
+  for (int i = 0, n = compile_threads(); i < n; ++i) std::thread(worker).detach();
 }
 
 std::string source(const std::vector<OpRec> &ops, bool c64, int A, int W) {

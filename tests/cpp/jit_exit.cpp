@@ -1,5 +1,5 @@
 // Process exit while specialised sweep kernels are compiling (CPU only: the
-// compiles need no GPU).  Queues compiles of several op-code sequences in
+// compiles, in helper processes, need no GPU).  Queues compiles of several op-code sequences in
 // the automatic mode, then returns from main half a second later, while
 // they run; the process must exit normally.  With `wait` as argument it first waits for the
 // compiles and prints how many compiled and failed.
@@ -40,7 +40,7 @@ int main(int argc, char **argv) {
     qs_sweep_jit_stats(&compiled, &failed, &launches);
     std::printf("compiled %lu failed %lu\n", (unsigned long)compiled, (unsigned long)failed);
   }
-  // let the compile threads get inside hipRTC before the exit
+  // let the compiles start before the exit
   std::this_thread::sleep_for(std::chrono::milliseconds(500));
   std::printf("main returns\n");
   if (argc > 2 && std::strcmp(argv[1], "status") == 0) return std::atoi(argv[2]);

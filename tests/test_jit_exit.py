@@ -1,10 +1,10 @@
-"""Process exit while sweep kernels compile at run time (hipRTC, no GPU
-needed): the library's exit handler waits for the running compiles (or,
-after its limit, ends the process with its exit status), and hipRTC's own
-exit-time teardown must come after it.  A handler registered
-before hipRTC's first use ran after that teardown, and a process that
-exited with compiles in flight crashed or hung.  The compiler cache is
-switched off so that every compile really runs."""
+"""Process exit while sweep kernels compile at run time (no GPU needed).
+Each kernel compiles with hipRTC in a helper process that the library
+starts; at exit the library stops the helpers still running and waits for
+none.  Compiling in the process itself went wrong at exit: hipRTC's
+exit-time teardown ran under compiles still in flight, and the process
+crashed or hung.  The compiler cache is switched off so that every compile
+really runs."""
 import os
 import subprocess
 import time
@@ -49,16 +49,14 @@ def test_exit_with_compiles_in_flight(prog):
         p, dt = _run(prog)
         assert p.returncode == 0, f"exit status {p.returncode}\n{p.stdout}{p.stderr}"
         assert "main returns" in p.stdout
-        # at most the two running compiles finish after main returns (the
-        # exit handler waits for them 5 s at most)
-        assert dt < 30.0, f"exit took {dt:.1f} s"
+        # the program sleeps 0.5 s; the exit itself waits for no compile
+        assert dt < 10.0, f"exit took {dt:.1f} s"
 
 
-def test_exit_status_when_compiles_are_cut_short(prog):
-    """With no time to wait (QS_SWEEP_JIT_EXIT_WAIT=0) a compile still running
-    at exit ends the process at once: stdout flushed, exit status kept."""
+def test_exit_status_kept(prog):
+    """The exit status of a program that exits with compiles running."""
     for status in ("0", "3"):
-        p, dt = _run(prog, "status", status, QS_SWEEP_JIT_EXIT_WAIT="0")
+        p, dt = _run(prog, "status", status)
         assert p.returncode == int(status), f"exit status {p.returncode}\n{p.stdout}{p.stderr}"
         assert "main returns" in p.stdout
-        assert dt < 30.0
+        assert dt < 10.0
