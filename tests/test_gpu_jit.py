@@ -89,3 +89,44 @@ def test_specialised_batched_sweeps_adjoint():
     assert after[2] > before[2]
     assert np.max(np.abs(res[2][0] - res[0][0])) < 1e-13
     assert np.max(np.abs(res[2][1] - res[0][1])) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_specialised_sweeps_30q_benchmark_circuit(dtype):
+    """The benchmark's own workload (30 qubits, 20 layers of the random
+    circuit) through the specialised kernels it is timed on, against the
+    generic kernel: one pass with the generic kernel queues the compiles
+    (automatic mode), a second register then runs every sweep specialised.
+    Same op functions in the same order, so the amplitudes must agree bit
+    for bit (4096 sampled indices) and the overlap must be 1."""
+    n = 30
+    w = circuits.random_circuit(n, 20, seed=2025, triples=8, measure=False)
+    idx = np.random.default_rng(7).integers(0, 1 << n, 4096, dtype=np.uint64)
+    with _Mode(0):
+        ra = Register.new([False] * n, dtype)
+        for op in w.ops:
+            ra.apply(op)
+        ra.sync()
+    with _Mode(1):
+        rq = Register.new([False] * n, dtype)
+        for op in w.ops:
+            rq.apply(op)
+        rq.sync()
+        rq.close()
+        assert _lib.load().qs_sweep_jit_wait() == 0
+        before = _stats()
+        rb = Register.new([False] * n, dtype)
+        for op in w.ops:
+            rb.apply(op)
+        rb.sync()
+        after = _stats()
+    sweeps = rb.stats()["sweeps"]
+    print(f"{dtype}: {after[2] - before[2]} specialised launches of {sweeps} sweeps", flush=True)
+    assert after[1] == before[1], "a specialised kernel failed to compile"
+    assert after[2] - before[2] == sweeps, "not every sweep ran specialised"
+    a, b = ra.amplitudes(idx), rb.amplitudes(idx)
+    assert np.array_equal(a, b), f"max diff {np.max(np.abs(a - b))}"
+    norm = ra.norm_sqr()
+    assert abs(ra.inner(rb) - norm) < 1e-12 * norm
+    ra.close()
+    rb.close()
