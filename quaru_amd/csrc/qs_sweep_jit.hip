@@ -88,7 +88,7 @@ Jit &jit() {
 const std::string &helper_path() {
   static const std::string path = [] {
     const char *e = std::getenv("QS_SWEEP_COMPILER");
-    if (e && *e) return std::string(e);
+    if (e && *e) return access(e, X_OK) == 0 ? std::string(e) : std::string();
     Dl_info info;
     if (!dladdr((void *)&helper_path, &info) || !info.dli_fname) return std::string();
     std::string lib = info.dli_fname;
@@ -352,6 +352,14 @@ hipFunction_t jit_kernel(const std::string &sig, const std::vector<OpRec> &ops, 
   auto it = j.table.find(sig);
   if (it == j.table.end()) {
     if (j.table.size() >= kMaxKernels) return nullptr;  // the generic kernel from here on
+    if (helper_path().empty()) {
+      // no compiler helper: the generic kernel from here on
+      std::fprintf(stderr,
+                   "quaru-hip: bin/qs_sweep_compile not found beside the library; "
+                   "sweeps use the generic kernel\n");
+      j.mode = 0;
+      return nullptr;
+    }
     start_workers(j);
     it = j.table.emplace(sig, Entry()).first;
     ++j.busy;
