@@ -132,6 +132,9 @@ bool compile(const std::string &src, std::vector<char> &code, std::string &log) 
   posix_spawn_file_actions_adddup2(&fa, in[0], 0);
   posix_spawn_file_actions_adddup2(&fa, out[1], 1);
   posix_spawn_file_actions_adddup2(&fa, err[1], 2);
+  // the helper inherits no other descriptor of this process (a device file
+  // among them)
+  posix_spawn_file_actions_addclosefrom_np(&fa, 3);
   posix_spawnattr_t at;
   posix_spawnattr_init(&at);
   sigset_t none;
