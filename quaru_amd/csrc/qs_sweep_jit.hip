@@ -50,7 +50,8 @@ const char *const kCompileOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++
                                        "-target-feature",       "-Xclang", "-packed-fp32-ops"};
 
 // Specialised kernels kept per process (a long run over ever-new circuits
-// stops compiling there and stays on the generic kernel).
+// stops compiling there and stays on the generic kernel); a code object is
+// 20-30 KB, so the table holds at most about 120 MB of them.
 constexpr size_t kMaxKernels = 4096;
 
 struct Entry {
