@@ -297,9 +297,14 @@ def adjoint_bench(n: int, layers: int, batch: int, fd_params: int, dtype: str):
 
 
 def measure_bench(reg: Register, n: int):
-    """`measure` (src/register.rs:291-342) on an n-qubit complex128 state: the
-    FP64 probability split reads the state once (2^n x 16 B); the collapse
-    reads the kept half and writes the whole state (1.5 x 2^n x 16 B).  Then
+    """`measure` (src/register.rs:291-342) on an n-qubit complex128 state,
+    three in a row (targets 0, n/2, n-1) and a synchronisation: the first
+    FP64 probability split reads the state (S = 2^n x 16 B); each later split
+    runs fused with the previous measure's collapse, which reads the kept
+    amplitudes (all of them for target 0, whose kept half is interleaved in
+    every line; half for the others) and writes the state; the last collapse
+    runs at the synchronisation.  S + 2S + 1.5S + 1.5S = 6S for three
+    measures, 2S per measure.  Then
     a full readout (measure_sequence: one histogram pass over the first 12
     qubits, a gather of the consistent amplitudes, one collapse), twice.  A measure
     returns once its outcome is known and leaves the collapse queued, so the
@@ -313,7 +318,7 @@ def measure_bench(reg: Register, n: int):
         reg.measure(target, 0.5)
     reg.sync()
     t_m = (time.perf_counter() - t0) / 3
-    byt = 2.5 * (1 << n) * amp
+    byt = 2.0 * (1 << n) * amp
     # the first readout also loads its kernels and sizes the register's work
     # buffer; the second (on the collapsed state: the same passes) is the
     # steady-state time
@@ -326,7 +331,8 @@ def measure_bench(reg: Register, n: int):
     return {"workload": f"{n}-qubit complex128 state",
             "measure": {"ms": t_m * 1e3, "achieved": byt / t_m / 1e9, "unit": "GB/s",
                         "frac": byt / t_m / HBM_PEAK, "bytes": byt,
-                        "kernels": "k_prob_split (FP64 partials) + k_collapse"},
+                        "kernels": "k_prob_split, then k_collapse_split (a collapse fused "
+                                   "with the next split), last k_collapse"},
             "measure_sequence_all_qubits_ms": t_seq[1] * 1e3,
             "measure_sequence_first_call_ms": t_seq[0] * 1e3}
 

@@ -100,6 +100,11 @@ hipError_t launch_prob_split(const void *psi, int n, int target, double *d_part,
                              hipStream_t s);
 template <typename R>
 hipError_t launch_collapse(void *psi, int n, int target, int keep, double total, hipStream_t s);
+// launch_collapse(ctarget, keep, total) and launch_prob_split(target) in one
+// pass, with the same {prob_0, prob_1} bits as the two in sequence.
+template <typename R>
+hipError_t launch_collapse_split(void *psi, int n, int ctarget, int keep, double total,
+                                 int target, double *d_part, double *d_out, hipStream_t s);
 // Measure and collapse with the outcome decided on the device (no host
 // round trip): d_probs (2 doubles) and d_coll (2 doubles) are work space, the
 // outcome (0 or 1) goes to *d_slot.

@@ -1057,8 +1057,25 @@ __global__ __launch_bounds__(kBlock) void k_prob_split(const typename V2<R>::T *
                                                        double *__restrict__ part) {
   __shared__ double sh[2][kBlock];
   double v[2] = {0.0, 0.0};
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
-       i += (uint64_t)gridDim.x * kBlock) {
+  // four loads in flight per trip; each thread still adds its amplitudes
+  // in index order (k_collapse_split relies on the same order)
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + 3 * S < len; i += 4 * S) {
+    typename V2<R>::T a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = psi[i + k * S];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double re = (double)a[k].x, im = (double)a[k].y;
+      const double p = re * re + im * im;
+      if (((i + k * S) >> target) & 1ull)
+        v[1] += p;
+      else
+        v[0] += p;
+    }
+  }
+  for (; i < len; i += S) {
     const auto a = psi[i];
     const double re = (double)a.x, im = (double)a.y;
     const double p = re * re + im * im;
@@ -1104,6 +1121,68 @@ __global__ __launch_bounds__(kBlock) void k_collapse(typename V2<R>::T *__restri
     }
     psi[i] = a;
   }
+}
+
+// The collapse of one measure fused with the probability split of the next
+// (src/register.rs:297-339 twice): amplitudes whose bit `ctarget` differs
+// from `keep` are zeroed, the rest divided by sqrt(total), and the squared
+// moduli of the values stored (after rounding to R) are summed by bit
+// `target`, over the same grid, in the same order and with the same block
+// sums as k_prob_split on the collapsed state, so {prob_0, prob_1} are the
+// same bits as a separate split would give.  One pass instead of two: the
+// split's full read of the collapsed state is saved.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_collapse_split(typename V2<R>::T *__restrict__ psi,
+                                                           uint64_t len, int ctarget, int keep,
+                                                           double sq, int target,
+                                                           double *__restrict__ part) {
+  using T = typename V2<R>::T;
+  __shared__ double sh[2][kBlock];
+  double v[2] = {0.0, 0.0};
+  auto one = [&](uint64_t i, T a) {
+    if ((int)((i >> ctarget) & 1ull) != keep) {
+      a = czero<T>();
+    } else {
+      a.x = (R)((double)a.x / sq);
+      a.y = (R)((double)a.y / sq);
+    }
+    psi[i] = a;
+    const double re = (double)a.x, im = (double)a.y;
+    const double p = re * re + im * im;
+    if ((i >> target) & 1ull)
+      v[1] += p;
+    else
+      v[0] += p;
+  };
+  // the trip structure and order of k_prob_split; only kept amplitudes are
+  // read
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + 3 * S < len; i += 4 * S) {
+    T a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      a[k] = (int)(((i + k * S) >> ctarget) & 1ull) == keep ? psi[i + k * S] : czero<T>();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) one(i + k * S, a[k]);
+  }
+  for (; i < len; i += S) one(i, (int)((i >> ctarget) & 1ull) == keep ? psi[i] : czero<T>());
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = v[0];
+    part[2 * blockIdx.x + 1] = v[1];
+  }
+}
+
+template <typename R>
+hipError_t launch_collapse_split(void *psi, int n, int ctarget, int keep, double total,
+                                 int target, double *d_part, double *d_out, hipStream_t s) {
+  const uint64_t len = 1ull << n;
+  const int g = reduce_grid(len);
+  hipLaunchKernelGGL(k_collapse_split<R>, dim3(g), dim3(kBlock), 0, s,
+                     (typename V2<R>::T *)psi, len, ctarget, keep, sqrt(total), target, d_part);
+  hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kBlock), 0, s, (const double *)d_part, g, d_out);
+  return hipGetLastError();
 }
 
 // Deferred measurement, decided on the device: probs = {prob_0, prob_1} from
@@ -1529,6 +1608,8 @@ hipError_t launch_chunk_sums(const void *psi, int n, uint64_t chunk, double *d_o
   template hipError_t launch_prob_split<R>(const void *, int, int, double *, double *,          \
                                            hipStream_t);                                         \
   template hipError_t launch_collapse<R>(void *, int, int, int, double, hipStream_t);           \
+  template hipError_t launch_collapse_split<R>(void *, int, int, int, double, int, double *,     \
+                                               double *, hipStream_t);                           \
   template hipError_t launch_measure_deferred<R>(void *, int, int, double, double *, double *,  \
                                                  double *, unsigned char *, hipStream_t);       \
   template hipError_t launch_norm<R>(const void *, int, double *, double *, hipStream_t);       \

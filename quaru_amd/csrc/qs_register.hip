@@ -85,6 +85,12 @@ struct qs_register {
   size_t ring_cap = 0, ring_off = 0;
   bool fusion = true;
   std::vector<Gate> pending;
+  // the collapse of the last measure, not yet launched: the next measure
+  // runs it fused with its own probability split; anything else launches it
+  // first (settle)
+  bool coll_pending = false;
+  int coll_target = 0, coll_keep = 0;
+  double coll_total = 1.0;
   uint64_t st_sweeps = 0, st_kernels = 0, st_gates = 0;
   uint64_t st_sweep_ops = 0, st_merged = 0;
   qs::SweepEngine *engine = nullptr;
@@ -547,9 +553,25 @@ hipError_t collect_marks(qs_register *r) {
   return hipSuccess;
 }
 
-// Apply every pending gate: through the sweep engine when fusion is on,
-// otherwise one kernel per gate.
+// Launch the pending collapse of the last measure, if any.
+int settle(qs_register *r) {
+  if (!r->coll_pending) return QS_OK;
+  r->coll_pending = false;
+  const hipError_t e =
+      r->dtype == QS_C128
+          ? qs::launch_collapse<double>(r->psi, r->n, r->coll_target, r->coll_keep,
+                                        r->coll_total, r->stream)
+          : qs::launch_collapse<float>(r->psi, r->n, r->coll_target, r->coll_keep, r->coll_total,
+                                       r->stream);
+  if (e != hipSuccess) return hip_fail(e, "collapse");
+  return QS_OK;
+}
+
+// Apply the pending collapse, then every pending gate: through the sweep
+// engine when fusion is on, otherwise one kernel per gate.
 int flush(qs_register *r) {
+  int rc = settle(r);
+  if (rc) return rc;
   if (r->pending.empty()) return QS_OK;
   std::vector<Gate> gates;
   gates.swap(r->pending);
@@ -608,18 +630,31 @@ hipError_t reduce_norm(qs_register *r, double *out) {
   return e;
 }
 
-// One measure (src/register.rs:297-342) on the device state.
+// One measure (src/register.rs:297-342) on the device state.  The split
+// reads the state (fused with the previous measure's collapse when that is
+// still pending); the collapse itself is left pending, for the next measure
+// to fuse or for the next other call to launch (settle).  Pending gates must
+// have been flushed.
 template <typename R>
 int measure_one(qs_register *r, size_t target, double x, int *result) {
   double *res_dev = r->d_red + 4 * qs::kReduceBlocks;
-  QS_HIP(qs::launch_prob_split<R>(r->psi, r->n, (int)target, r->d_red, res_dev, r->stream));
+  if (r->coll_pending) {
+    r->coll_pending = false;
+    QS_HIP(qs::launch_collapse_split<R>(r->psi, r->n, r->coll_target, r->coll_keep,
+                                        r->coll_total, (int)target, r->d_red, res_dev,
+                                        r->stream));
+  } else {
+    QS_HIP(qs::launch_prob_split<R>(r->psi, r->n, (int)target, r->d_red, res_dev, r->stream));
+  }
   QS_HIP(hipMemcpyAsync(r->h_red, res_dev, 2 * sizeof(double), hipMemcpyDeviceToHost,
                         r->stream));
   QS_HIP(hipStreamSynchronize(r->stream));
   const double p0 = r->h_red[0], p1 = r->h_red[1];
   const int res = x > p0;
-  const double total = res ? p1 : p0;
-  QS_HIP(qs::launch_collapse<R>(r->psi, r->n, (int)target, res, total, r->stream));
+  r->coll_pending = true;
+  r->coll_target = (int)target;
+  r->coll_keep = res;
+  r->coll_total = res ? p1 : p0;
   *result = res;
   return QS_OK;
 }
@@ -1112,8 +1147,12 @@ int qs_apply_all(qs_register *r, const double *matrix, size_t rows, size_t cols,
 int qs_measure(qs_register *r, size_t target, double uniform, int *result) {
   if (!r || !result) return fail(QS_ERR_ARGUMENT, 0, 0, "NULL argument");
   if (target >= (size_t)r->n) return fail(QS_INVALID_TARGET, target, 0, "target out of range");
-  int rc = flush(r);
-  if (rc) return rc;
+  // a collapse left by the previous measure stays pending (the split fuses
+  // it) unless gates were applied since: then it goes first, then the gates
+  if (!r->pending.empty()) {
+    int rc = flush(r);
+    if (rc) return rc;
+  }
   return measure_dispatch(r, target, uniform, result);
 }
 

@@ -412,6 +412,52 @@ def test_measure_matches_oracle(dtype):
             assert maxdiff(r.state, ref) < tol
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_measure_chain_with_pending_collapses(dtype):
+    """Measures one after another (each collapse runs fused with the next
+    measure's probability split) mixed with gates, readouts, a clone and a
+    full readout, each of which must first see the collapse applied: the
+    outcomes equal the oracle's on the same uniforms, the state after every
+    step too (src/register.rs:291-342)."""
+    rng = np.random.default_rng(29)
+    n = 13
+    psi = random_state(rng, n)
+    tol = TOL if dtype == "c128" else 1e-6
+    r = Register.new([False] * n, dtype)
+    r.state = psi
+    ref = psi.copy()
+
+    def meas(t, x):
+        nonlocal ref
+        got = r.measure(t, x)
+        _, want, ref, _ = oracle.try_measure(ref, t, x)
+        assert got == want, (t, x)
+
+    meas(0, 0.31)
+    meas(7, 0.62)            # fused with the collapse on qubit 0
+    meas(12, 0.45)           # fused again
+    assert maxdiff(r.amplitudes(np.arange(64)), ref[:64]) < tol   # readout settles
+    meas(3, 0.77)
+    meas(3, 0.05)            # the same target again: certain outcome
+    h = ops.hadamard(5)
+    r.apply(h)               # gate after a pending collapse: collapse first
+    ref = oracle.apply_fast(ref, h.matrix(), h.targets())
+    meas(5, 0.52)
+    assert abs(r.norm_sqr() - float(np.vdot(ref, ref).real)) < (1e-9 if dtype == "c128" else 1e-6)
+    meas(9, 0.2)
+    c = r.clone()            # the clone sees the collapsed state
+    assert maxdiff(c.state, ref) < tol
+    c.close()
+    meas(1, 0.66)
+    u = [0.4, 0.9, 0.1]
+    idx = r.measure_sequence([2, 4, 6], u)     # settles, then its own pass
+    ridx, ref = oracle.measure_sequence(ref, [2, 4, 6], u)
+    assert idx == ridx
+    meas(11, 0.5)
+    assert maxdiff(r.state, ref) < tol
+    r.close()
+
+
 def test_measure_basis_and_bell():
     rng = np.random.default_rng(3)
     for n in (4, 8):
