@@ -346,10 +346,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
   // most `sweep_high_lanes` of them to lane bits.
   const int hq = sweep_page_qubit(run.dtype_c64);
   // Each dense gate on a wave bit costs an exchange through LDS with four
-  // barriers.  Measured on the 30-qubit complex128 circuit: with both wave
-  // bits open to dense gates, 62 sweeps of 8.1-8.2 ms; with one, 70 sweeps of
-  // 7.2 ms (about the same gates/s, 0.60 instead of 0.53 of HBM peak per
-  // sweep); with none, 81 sweeps of 6.8 ms.
+  // barriers; how many wave bits take dense gates: sweep_dense_wave_bits.
   const int WD = std::min(W, sweep_dense_wave_bits());
   const TileShape ts{A, lowb, A + sweep_high_lanes() + WD,
                      hq >= 64 ? 0ull : (allq & ~((1ull << hq) - 1ull)), W, WD};
@@ -699,9 +696,12 @@ bool sweep_wave_bits(bool c64) {
   return c64 ? on64 : on128;
 }
 
-// Wave bits open to dense gates (QS_SWEEP_DENSE_WAVES, 0..2, default 1).
+// Wave bits open to dense gates (QS_SWEEP_DENSE_WAVES, 0..2, default 2).
+// With the specialised kernels both pay on the 30-qubit circuit: 62 sweeps
+// of 6.94 ms (3065 gates/s) against 70 of 6.58 ms (2847) with one open and
+// 81 of 6.60 ms (2467) with none; complex64 5717 / 5528 / 5073 gates/s.
 int sweep_dense_wave_bits() {
-  static const int d = std::min(kWaveBits, std::max(0, env_int("QS_SWEEP_DENSE_WAVES", 1)));
+  static const int d = std::min(kWaveBits, std::max(0, env_int("QS_SWEEP_DENSE_WAVES", 2)));
   return d;
 }
 

@@ -160,7 +160,7 @@ def test_batched_rotations_plan(tmp_path):
                           m_all.ctypes.data, t_all.ctypes.data, ks_a.ctypes.data, None, None,
                           bm_a.ctypes.data, n, stats.ctypes.data, None, err, 512)
         assert rc == 0, err.value.decode()
-        assert stats[0] >= 2 and stats[1] == 0, stats  # sweeps only
+        assert stats[0] >= 1 and stats[1] == 0, stats  # sweeps only
         # every RZ(q) folds into the RY(q) before it (state by state)
         assert stats[3] == layers * n, stats
         for b in range(B):
