@@ -185,7 +185,7 @@ def cpu_baseline(w: circuits.Workload, threads: int):
     del scratch, state
     layer_gates = n + (n - 1) + (8 if tof is not cn else 0)
     layer_omp = n * per["haar"] + (n - 1) * per["cnot"] + (layer_gates - (2 * n - 1)) * per["toffoli"]
-    # config 2's size: 26 qubits, 20 layers = 520 Haar + 500 CNOT gates
+    # the 26-qubit random circuit: 20 layers = 520 Haar + 500 CNOT gates
     n26 = 26
     s26 = np.zeros(1 << n26, dtype=np.complex128)
     s26[0] = 1.0
@@ -215,7 +215,7 @@ def cpu_baseline(w: circuits.Workload, threads: int):
                                 f"weighted by the per-layer gate mix, on every core this "
                                 f"process may run on ({threads} of nproc {os.cpu_count()})",
                       "seconds": per},
-        "config2_26q": {"value": 1020.0 / t26_circuit, "unit": "gates/s", "cores": 1,
+        "random_26q": {"value": 1020.0 / t26_circuit, "unit": "gates/s", "cores": 1,
                         "sample": "one Haar and one CNOT gate of the 26-qubit random circuit "
                                   "(the oracle's try_apply_fast, one thread), extrapolated "
                                   "to its 520 Haar + 500 CNOT gates",

@@ -22,8 +22,8 @@
  *       (the 1e-6 tolerance and the InvalidQubit(i) payload).
  *   orc_try_apply_fast: test_apply_fast_matches_dense_and_tensordot (against
  *       orc_try_apply, a different algorithm, and a numpy tensordot),
- *       test_transposed_matrix_is_caught, test_golden_config1_12q and
- *       test_golden_config3_mini (numpy goldens), test_grover_closed_form,
+ *       test_transposed_matrix_is_caught, test_golden_random_12q and
+ *       test_golden_toffoli_mini (numpy goldens), test_grover_closed_form,
  *       test_qft_add_identity, test_u_gate_modmul_exhaustive
  *       (examples/shors.rs:434-503), test_toffoli_cnx.
  *   orc_try_apply: test_apply_fast_matches_dense_and_tensordot,
@@ -34,7 +34,7 @@
  *       test_measure_rule_and_collapse (x > prob_0, strict, and the
  *       normalisation), test_bell_pairs_measure_equal,
  *       test_teleportation_outcome_independent, test_superdense_coding.
- *   orc_measure_sequence: test_golden_config1_12q (sampled index 3733 and the
+ *   orc_measure_sequence: test_golden_random_12q (sampled index 3733 and the
  *       collapsed state of the numpy golden).
  *   orc_apply_fast_omp (qs_oracle_omp.c): test_omp_variant_matches_scalar
  *       (bit-identical to orc_try_apply_fast).

@@ -1,4 +1,4 @@
-"""Seeded synthetic workloads: the random-circuit, Grover and ansatz configs.
+"""Seeded synthetic workloads: the random-circuit, Grover and ansatz workloads.
 
 Everything here is generated on the host in FP64 from one documented
 counter-free generator, so the GPU engine and the CPU oracle receive the
@@ -13,9 +13,9 @@ very same gate matrices.
   the QR factor with a positive real diagonal R, which is exactly Mezzadri's
   phase-corrected Q (Q . diag(R_ii/|R_ii|)), the Haar measure on U(2).
 
-Layer structure of the random circuits: per layer a
+Layer structure of the random circuits (the benchmark workloads): per layer a
 fresh Haar unitary on every qubit 0..n-1, then ``cnot(q, q+1)`` for
-q = 0..n-2, then (config 3) ``triples`` three-qubit gates on seeded distinct
+q = 0..n-2, then (when asked for) ``triples`` three-qubit gates on seeded distinct
 qubit triples (a, b, c): even gate index -> Toffoli ``cnx([a, b], c)``, odd
 -> ``cnz([a, b], c)``.
 """
@@ -70,7 +70,7 @@ def haar_2x2(rng: SplitMix64) -> np.ndarray:
 
 @dataclass
 class Workload:
-    """A gate list plus what the configs attach to it."""
+    """A gate list plus what the workloads attach to it."""
     n: int
     ops: List[ops.Operation]
     seed: int

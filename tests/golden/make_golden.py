@@ -9,11 +9,11 @@ src/register.rs:165-176).  Measurement follows try_measure
 (src/register.rs:297-342): res = x > prob_0, collapse, divide by sqrt.
 
 Fixtures:
-  c1_12q.npz    BASELINE config 1: 12 qubits, 20 layers of Haar 2x2 on
+  c1_12q.npz    the 12-qubit random circuit: 20 layers of Haar 2x2 on
                 every qubit + cnot(q, q+1) ladder, seed 2024; final
                 pre-measure amplitudes, the uniforms, and the index sampled
                 by measuring qubits 0..11 in order.
-  c3_mini.npz   config-3 layer structure at 10 qubits, 3 layers, 4
+  c3_mini.npz   the Toffoli/CCZ layer structure at 10 qubits, 3 layers, 4
                 Toffoli/CCZ per layer, seed 7 (Toffoli / CCZ coverage).
   adj_4q.npz    4-qubit, 2-layer RY/RZ + CZ-ring ansatz: energy and exact
                 parameter-shift gradients of H = sum Z_i + 0.5 sum Z_i Z_i+1

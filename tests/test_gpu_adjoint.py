@@ -270,10 +270,10 @@ def oracle_energy_from(ans, theta, state, layer, masks, coeffs):
     return oracle.expect_zstrings(s, masks, coeffs)
 
 
-_CONFIG5 = {}
+_ANSATZ20 = {}
 
 
-def _config5_reference():
+def _ansatz20_reference():
     """The differentiation workload at its own size: 20 qubits, 10 layers of
     RY/RZ on every qubit and a CZ ring (400 parameters), H = sum Z_i + 0.5
     sum Z_i Z_{i+1}, a batch of 64 seeded parameter sets.  FP64 checks:
@@ -281,13 +281,13 @@ def _config5_reference():
     of 16 seeded parameters of set 1 on the complex128 engine; oracle
     parameter shift of 4 parameters of set 1 in layers 9, 8, 7 and 5
     (started from the oracle's cached layer states)."""
-    if _CONFIG5:
-        return _CONFIG5
+    if _ANSATZ20:
+        return _ANSATZ20
     n, layers, batch = 20, 10, 64
     ans = circuits.hardware_efficient_ansatz(n, layers)
     masks, coeffs = circuits.zz_chain_observable(n)
     th = circuits.seeded_thetas(batch, ans.nparams, seed=5)
-    print("config 5: oracle forward of sets 0 and 1", flush=True)
+    print("20-qubit ansatz: oracle forward of sets 0 and 1", flush=True)
     e0 = oracle_energy_from(ans, th[0], oracle.new([False] * n), 0, masks, coeffs)
     states1 = oracle_layer_states(ans, th[1])
     e1 = oracle.expect_zstrings(states1[-1], masks, coeffs)
@@ -300,7 +300,7 @@ def _config5_reference():
         tm[p] -= math.pi / 2
         ps[p] = 0.5 * (oracle_energy_from(ans, tp, states1[layer], layer, masks, coeffs)
                        - oracle_energy_from(ans, tm, states1[layer], layer, masks, coeffs))
-        print(f"config 5: parameter shift of parameter {p} done", flush=True)
+        print(f"20-qubit ansatz: parameter shift of parameter {p} done", flush=True)
     rng = np.random.default_rng(55)
     fd_params = np.sort(rng.choice(ans.nparams, 16, replace=False))
     h = 1e-4
@@ -310,22 +310,22 @@ def _config5_reference():
     minus[np.arange(fd_params.size), fd_params] -= h
     ex = Expectation(ParamCircuit.from_ansatz(ans), masks, coeffs, dtype="c128")
     fd = (ex.energies(plus) - ex.energies(minus)) / (2 * h)
-    _CONFIG5.update(ans=ans, masks=masks, coeffs=coeffs, th=th, e=(e0, e1), ps=ps,
+    _ANSATZ20.update(ans=ans, masks=masks, coeffs=coeffs, th=th, e=(e0, e1), ps=ps,
                     fd_params=fd_params, fd=fd)
-    return _CONFIG5
+    return _ANSATZ20
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-def test_config5_full_size(dtype):
+def test_ansatz20_full_size(dtype):
     """20 qubits x 10 layers x batch 64, checkpoint every 2 layers, in both
     precisions: energies against the oracle, gradients against central FD
     (FP64, h = 1e-4) and oracle parameter shift, all within 1e-5 (complex128
     energies within 1e-10)."""
-    ref = _config5_reference()
+    ref = _ansatz20_reference()
     ans, th = ref["ans"], ref["th"]
     ex = Expectation(ParamCircuit.from_ansatz(ans), ref["masks"], ref["coeffs"], dtype=dtype,
                      checkpoint_every=2)
-    print(f"config 5 {dtype}: forward + backward, batch {th.shape[0]}", flush=True)
+    print(f"20-qubit ansatz {dtype}: forward + backward, batch {th.shape[0]}", flush=True)
     e = ex.forward(th)
     g = ex.backward()
     assert e.shape == (64,) and g.shape == (64, 400)

@@ -62,7 +62,7 @@ def _prefix26():
     return opl, ref
 
 
-def test_config2_26q_prefix_matches_oracle():
+def test_random_26q_prefix_matches_oracle():
     n = 26
     opl, ref = _prefix26()
     r = _zero(n)
@@ -91,7 +91,7 @@ def test_30q_high_qubits_match_oracle():
     assert abs(got[0]) < 0.5  # the state moved away from |0...0>
 
 
-def test_config2_26q_mirror_returns_to_zero():
+def test_random_26q_mirror_returns_to_zero():
     n = 26
     w = circuits.random_circuit(n, 20, seed=2026, measure=False)
     assert len(w.ops) == 26 * 20 + 25 * 20
@@ -159,7 +159,7 @@ def test_streaming_per_gate_kernels_26q(dtype):
         assert inf <= 1e-6
 
 
-def test_config3_30q_c64_vs_c128_and_mirror():
+def test_random_30q_c64_vs_c128_and_mirror():
     n = 30
     w = circuits.random_circuit(n, 20, seed=2025, triples=8, measure=False)
     print(f"30q: {len(w.ops)} gates", flush=True)
@@ -260,7 +260,7 @@ def test_grover_closed_form_register_small():
 
 
 @pytest.mark.parametrize("fused", [True, False])
-def test_config4_32q_grover_c128(fused):
+def test_grover_32q_c128(fused):
     """fused: every Hadamard layer as 5-qubit 32 x 32 blocks, run by the block
     kernel (fusion off: with fusion on the engine would apply these separable
     blocks as their one-qubit factors through the sweeps, which the second
@@ -285,7 +285,7 @@ def test_config4_32q_grover_c128(fused):
     assert r.equals(e, TOL)
 
 
-def test_config4_32q_grover_c64():
+def test_grover_32q_c64():
     n, t = 32, 12
     w, opl = circuits.grover_operations(n, t, seed=4, fused=True)
     r = _zero(n, "c64").set_fusion(False)  # the 5-qubit blocks on the block kernel

@@ -1,7 +1,7 @@
 """GPU parity: the HIP register against the CPU oracle.
 
 Bar: complex128 results within 1e-12 per amplitude of the oracle (the
-north-star tolerance for the drop-in); complex64 within its own rounding
+tolerance of the drop-in); complex64 within its own rounding
 (infidelity against the FP64 oracle <= 1e-6).  Indices and outcomes are
 compared exactly.  Every case runs with gate fusion on (the default sweep
 engine) and off (one kernel per gate).
@@ -227,7 +227,7 @@ def test_gate_zoo_parity_c64(n):
     assert maxdiff(got, ref) < 1e-5
 
 
-# ---- the sweep engine on random circuits (configs 1 and 3 structure) -------------
+# ---- the sweep engine on random circuits (the benchmark layer structure) ---------
 
 @pytest.mark.parametrize("n,layers,triples", [(12, 4, 0), (14, 3, 4), (16, 2, 8), (18, 2, 8)])
 def test_random_circuit_sweeps(n, layers, triples):
@@ -345,7 +345,7 @@ def test_sweep_merged_ops_match_oracle(n, dtype):
         assert infid(r.state, ref) < 1e-6
 
 
-def test_golden_config1_on_gpu(golden):
+def test_golden_random_12q_on_gpu(golden):
     g = golden("c1_12q.npz")
     w = circuits.random_circuit(12, 20, seed=int(g["seed"]))
     for fusion in (True, False):
@@ -358,7 +358,7 @@ def test_golden_config1_on_gpu(golden):
         assert maxdiff(r.state, g["collapsed"]) < TOL
 
 
-def test_golden_config1_c64(golden):
+def test_golden_random_12q_c64(golden):
     g = golden("c1_12q.npz")
     w = circuits.random_circuit(12, 20, seed=int(g["seed"]))
     r = Register.new([False] * 12, "c64")
@@ -367,7 +367,7 @@ def test_golden_config1_c64(golden):
     assert infid(r.state, g["amplitudes"]) < 1e-6
 
 
-def test_golden_config3_mini(golden):
+def test_golden_toffoli_mini(golden):
     g = golden("c3_mini.npz")
     w = circuits.random_circuit(10, 3, seed=7, triples=4, measure=False)
     r = Register.new([False] * 10)

@@ -305,7 +305,7 @@ def test_superdense_coding():
 
 # ---- golden fixtures ------------------------------------------------------------------
 
-def test_golden_config1_12q(golden):
+def test_golden_random_12q(golden):
     g = golden("c1_12q.npz")
     w = circuits.random_circuit(12, 20, seed=int(g["seed"]))
     assert np.array_equal(np.asarray(w.uniforms), g["uniforms"])
@@ -316,7 +316,7 @@ def test_golden_config1_12q(golden):
     assert np.max(np.abs(coll - g["collapsed"])) < 1e-12
 
 
-def test_golden_config3_mini(golden):
+def test_golden_toffoli_mini(golden):
     g = golden("c3_mini.npz")
     w = circuits.random_circuit(10, 3, seed=7, triples=4, measure=False)
     s = run(oracle.new([False] * 10), w.ops)
