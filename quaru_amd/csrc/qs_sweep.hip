@@ -286,7 +286,7 @@ void encode_mux(const SweepOp &o, const int *slot, uint64_t reg_qubits, uint64_t
 
 template <typename R>
 hipError_t launch_sweep_a(int A, int W, SweepRunner &run, const void *d_blob, uint64_t ntiles,
-                          const std::vector<OpRec> &ops) {
+                          const std::vector<OpRec> &ops, const uint64_t *gb) {
   const bool xcd = sweep_xcd_chunks();
   // a kernel specialised to this op-code sequence, when one is compiled
   // (32-bit indices only; in the automatic mode only from 22 qubits, where a
@@ -294,7 +294,8 @@ hipError_t launch_sweep_a(int A, int W, SweepRunner &run, const void *d_blob, ui
   const int jm = run.n <= 32 ? jit_mode() : 0;
   if (jm == 2 || (jm == 1 && run.n >= 22)) {
     const bool c64 = std::is_same<R, float>::value;
-    hipFunction_t fn = jit_kernel(jit_signature(ops, c64, A, W), ops, c64, A, W);
+    const std::vector<int> rp = jit_register_patterns(ops, gb, A);
+    hipFunction_t fn = jit_kernel(jit_signature(ops, c64, A, W, rp), ops, c64, A, W, rp);
     if (fn) {
       uint64_t grid;
       uint32_t xcd_chunk;
@@ -649,7 +650,7 @@ hipError_t execute_t(SweepRunner &run, const std::vector<Gate> &gates, uint64_t 
     hipError_t e = run.upload(blob.data(), blob.size(), &d_blob);
     if (e != hipSuccess) return e;
     if (run.mark) run.mark(true);
-    e = launch_sweep_a<R>(A, W, run, d_blob, h.ntiles, ops);
+    e = launch_sweep_a<R>(A, W, run, d_blob, h.ntiles, ops, h.gb);
     if (run.mark) run.mark(false);
     if (e != hipSuccess) return e;
     ++*sweeps;

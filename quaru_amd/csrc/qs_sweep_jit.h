@@ -30,15 +30,23 @@ namespace sweep {
 
 struct OpRec;
 
+// Each op's pattern on the register slots, in the form the generated
+// kernel takes it as a constant (make_pred in qs_sweep_kernel.h: slot mask
+// in bits 0..7, slot values in bits 8..15), from the slots' qubits gb[0..A).
+std::vector<int> jit_register_patterns(const std::vector<OpRec> &ops, const uint64_t *gb, int A);
+
 // The key of a specialised kernel: precision, register bits, wave bits and
-// each op's code (and its fail code, and whether it is batched).
-std::string jit_signature(const std::vector<OpRec> &ops, bool c64, int A, int W);
+// each op's code (and its fail code, whether it is batched, and its
+// register pattern when `rp` is given).
+std::string jit_signature(const std::vector<OpRec> &ops, bool c64, int A, int W,
+                          const std::vector<int> &rp = {});
 
 // The compiled kernel for `sig` on the current device, or nullptr (not
 // compiled yet, compiling, off or failed).  Queues the compile of a new
-// signature; in mode 2 waits for it.
+// signature; in mode 2 waits for it.  `rp` (empty: patterns read at run
+// time) as for jit_signature.
 hipFunction_t jit_kernel(const std::string &sig, const std::vector<OpRec> &ops, bool c64, int A,
-                         int W);
+                         int W, const std::vector<int> &rp = {});
 
 int jit_mode();
 int jit_set_mode(int mode);  // returns the previous mode

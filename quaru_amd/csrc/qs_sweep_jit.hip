@@ -290,7 +290,8 @@ void start_workers(Jit &j) {
   for (int i = 0, n = compile_threads(); i < n; ++i) std::thread(worker).detach();
 }
 
-std::string source(const std::vector<OpRec> &ops, bool c64, int A, int W) {
+std::string source(const std::vector<OpRec> &ops, bool c64, int A, int W,
+                   const std::vector<int> &rp) {
   std::ostringstream o;
   const char *R = c64 ? "float" : "double";
   const int waves = (A <= 4 || c64) ? 3 : 1;
@@ -307,23 +308,24 @@ std::string source(const std::vector<OpRec> &ops, bool c64, int A, int W) {
        "    uint32_t code = 0;\n";
   for (size_t i = 0; i < ops.size(); ++i) {
     const OpRec &op = ops[i];
+    const int rpi = rp.empty() ? -1 : rp[i];
     const bool uni = op.umask != 0;
     if (op.code == CODE_NONE && (!uni || op.code_fail == CODE_NONE)) continue;
     if (uni || op.bmask) {
       o << "    code = op_setup<R, I>(ops[" << i << "], b.pool, c, m);\n";
       if (uni && op.code_fail != op.code) {
         if (op.code != CODE_NONE)
-          o << "    if (code == " << op.code << "u) apply_code<" << op.code << "u, A, W>(a, ops[" << i
+          o << "    if (code == " << op.code << "u) apply_code<" << op.code << "u, A, W, " << rpi << ">(a, ops[" << i
             << "], m, c, g);\n";
         if (op.code_fail != CODE_NONE)
           o << "    if (code == " << op.code_fail << "u) apply_code<" << op.code_fail
-            << "u, A, W>(a, ops[" << i << "], m, c, g);\n";
+            << "u, A, W, " << rpi << ">(a, ops[" << i << "], m, c, g);\n";
       } else {
-        o << "    apply_code<" << op.code << "u, A, W>(a, ops[" << i << "], m, c, g);\n";
+        o << "    apply_code<" << op.code << "u, A, W, " << rpi << ">(a, ops[" << i << "], m, c, g);\n";
       }
     } else {
       o << "    m = b.pool + ops[" << i << "].moff;\n";
-      o << "    apply_code<" << op.code << "u, A, W>(a, ops[" << i << "], m, c, g);\n";
+      o << "    apply_code<" << op.code << "u, A, W, " << rpi << ">(a, ops[" << i << "], m, c, g);\n";
     }
   }
   o << "    (void)code;\n    (void)m;\n  }\n};\n"
@@ -336,20 +338,36 @@ std::string source(const std::vector<OpRec> &ops, bool c64, int A, int W) {
 
 }  // namespace
 
-std::string jit_signature(const std::vector<OpRec> &ops, bool c64, int A, int W) {
+std::vector<int> jit_register_patterns(const std::vector<OpRec> &ops, const uint64_t *gb, int A) {
+  std::vector<int> rp(ops.size(), 0);
+  for (size_t i = 0; i < ops.size(); ++i) {
+    int rm = 0, rv = 0;
+    for (int j = 0; j < A; ++j) {
+      if (ops[i].cmask & gb[j]) rm |= 1 << j;
+      if (ops[i].cval & gb[j]) rv |= 1 << j;
+    }
+    rp[i] = rm | (rv << 8);
+  }
+  return rp;
+}
+
+std::string jit_signature(const std::vector<OpRec> &ops, bool c64, int A, int W,
+                          const std::vector<int> &rp) {
   std::string s = c64 ? "f" : "d";
   s += std::to_string(A) + "w" + std::to_string(W) + ":";
-  for (const OpRec &op : ops) {
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const OpRec &op = ops[i];
     s += std::to_string(op.code);
     if (op.umask) s += "u" + std::to_string(op.code_fail);
     if (op.bmask) s += "b";
+    if (!rp.empty()) s += "r" + std::to_string(rp[i]);
     s += ",";
   }
   return s;
 }
 
 hipFunction_t jit_kernel(const std::string &sig, const std::vector<OpRec> &ops, bool c64, int A,
-                         int W) {
+                         int W, const std::vector<int> &rp) {
   Jit &j = jit();
   std::unique_lock<std::mutex> lk(j.mu);
   if (j.mode == 0 || j.stopping) return nullptr;
@@ -367,7 +385,7 @@ hipFunction_t jit_kernel(const std::string &sig, const std::vector<OpRec> &ops, 
     start_workers(j);
     it = j.table.emplace(sig, Entry()).first;
     ++j.busy;
-    j.queue.emplace_back(sig, source(ops, c64, A, W));
+    j.queue.emplace_back(sig, source(ops, c64, A, W, rp));
     j.work.notify_one();
   }
   if (j.mode == 2) j.done.wait(lk, [&] { return it->second.state != 0; });

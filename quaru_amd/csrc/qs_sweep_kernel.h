@@ -230,28 +230,41 @@ struct Ctx {
   int wave;  // wave index in the workgroup
 };
 
-// on(r) = lane_ok && ((goff(r) & cmr) == cvr): the second factor is uniform.
+// on(r) = lane_ok && ((r & rm) == rv): the second factor is uniform.
 template <typename I>
 struct Pred {
   bool lane_ok;
-  I cmr, cvr;
+  uint32_t rm, rv;  // the pattern on the register slots: register r meets it if (r & rm) == rv
 };
 
-template <int A, typename I>
+// RP >= 0: the register-slot pattern known when the kernel was generated
+// (rm in bits 0..7, rv in bits 8..15), so that the per-register tests below
+// fold away; RP < 0: taken from the op's pattern and the slots' qubits.
+template <int A, int RP, typename I>
 __device__ __forceinline__ Pred<I> make_pred(const QS_CONST OpRec &op, const Ctx<I> &c,
                                              const Regs<A, I> &g) {
   const I cm = (I)op.cmask, cv = (I)op.cval;
   const I cmo = cm & ~g.mask;
   Pred<I> p;
-  p.cmr = cm & g.mask;
-  p.cvr = cv & g.mask;
+  if constexpr (RP >= 0) {
+    p.rm = (uint32_t)RP & 0xFFu;
+    p.rv = ((uint32_t)RP >> 8) & 0xFFu;
+  } else {
+    p.rm = 0;
+    p.rv = 0;
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+      if (cm & g.gb[j]) p.rm |= 1u << j;
+      if (cv & g.gb[j]) p.rv |= 1u << j;
+    }
+  }
   p.lane_ok = (c.gbase & cmo) == (cv & cmo);
   return p;
 }
 
 template <int A, typename I>
-__device__ __forceinline__ bool pred_on(const Pred<I> &p, const Regs<A, I> &g, int r) {
-  return p.lane_ok && ((goff<A>(g, r) & p.cmr) == p.cvr);
+__device__ __forceinline__ bool pred_on(const Pred<I> &p, const Regs<A, I> &, int r) {
+  return p.lane_ok && (((uint32_t)r & p.rm) == p.rv);
 }
 
 // Partner lane's value (lane ^ 2^K) by VALU cross-lane instructions: DPP
@@ -415,13 +428,13 @@ __device__ __forceinline__ float sld(const QS_CONST float *p) {
 }
 
 // 2x2 on register slot J; kCtl selects the controlled form.
-template <int J, bool kCtl, int A, typename R, typename I>
+template <int J, bool kCtl, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_1q_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                           const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
   const R m0r = m[0], m0i = m[1], m1r = m[2], m1i = m[3];
   const R m2r = m[4], m2i = m[5], m3r = m[6], m3i = m[7];
   Pred<I> p{};
-  if constexpr (kCtl) p = make_pred<A>(op, c, g);
+  if constexpr (kCtl) p = make_pred<A, RP>(op, c, g);
   constexpr int bj = 1 << J;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
@@ -485,13 +498,13 @@ __device__ __forceinline__ void lane_2x2_swapped(Amp<R, 1 << A> &a, bool hi, Coe
 // 2x2 on a lane qubit: a lane with the bit clear holds x and gets the
 // partner's y: out = m00 x + m01 y; a lane with the bit set holds y and
 // computes out = m10 x + m11 y from the partner's x.
-template <bool kCtl, int K, int A, typename R, typename I>
+template <bool kCtl, int K, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                            const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
   const bool hi = (c.lane >> K) & 1;
   if constexpr (K >= 4) {
     Pred<I> p{};
-    if constexpr (kCtl) p = make_pred<A>(op, c, g);
+    if constexpr (kCtl) p = make_pred<A, RP>(op, c, g);
     lane_2x2_swapped<K, 0, A>(
         a, hi,
         [&](int, R *t) {
@@ -513,7 +526,7 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
   const R sr = hi ? (ord ? mv[4] : mv[6]) : mv[0], si = hi ? (ord ? mv[5] : mv[7]) : mv[1];
   const R pr = hi ? (ord ? mv[6] : mv[4]) : mv[2], pi = hi ? (ord ? mv[7] : mv[5]) : mv[3];
   Pred<I> p{};
-  if constexpr (kCtl) p = make_pred<A>(op, c, g);
+  if constexpr (kCtl) p = make_pred<A, RP>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const R xr = a.re[r], xi = a.im[r];
@@ -552,11 +565,11 @@ __device__ __forceinline__ void mat2(R &ur, R &ui, R &wr, R &wi, const R *tr, co
 // fails, m[4..7] where it holds.  kRegPattern: the pattern lies on register
 // slots only, so the choice is uniform per register pair (scalar selects);
 // otherwise it lies on no register slot and each lane chooses once.
-template <int J, bool kRegPattern, int A, typename R, typename I>
+template <int J, bool kRegPattern, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                            const QS_CONST R *m, const Ctx<I> &c,
                                            const Regs<A, I> &g) {
-  const Pred<I> p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A, RP>(op, c, g);
   constexpr int bj = 1 << J;
   R lr[4], li[4];
   // both matrices on scalar loads once; register patterns choose per pair
@@ -582,7 +595,7 @@ __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpR
     if ((r >> J) & 1) continue;
     R tr[4], ti[4];
     if constexpr (kRegPattern) {
-      const bool hold = (goff<A>(g, r) & p.cmr) == p.cvr;  // the same for the whole wave
+      const bool hold = ((uint32_t)r & p.rm) == p.rv;  // the same for the whole wave
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         tr[k] = hold ? v1r[k] : v0r[k];
@@ -606,11 +619,11 @@ __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpR
 
 // Multiplexed 2x2 on a lane qubit; the pattern has no register-slot bit, so
 // each lane picks its matrix row once for the whole op.
-template <int K, int A, typename R, typename I>
+template <int K, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                             const QS_CONST R *m, const Ctx<I> &c,
                                             const Regs<A, I> &g) {
-  const Pred<I> p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A, RP>(op, c, g);
   const bool hi = (c.lane >> K) & 1;
   // this lane's output row: the coefficient of its own amplitude (s) and of
   // the partner's (p); selects keep the matrix on scalar loads
@@ -659,7 +672,7 @@ __device__ __forceinline__ void op_mux_lane(Amp<R, 1 << A> &a, const QS_CONST Op
 // the register pairs with slot bit JC set use one matrix, the others the
 // other, so each lane picks its two row pairs once and every register's
 // choice is fixed at compile time.
-template <int JC, int K, int A, typename R, typename I>
+template <int JC, int K, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_mux_lane_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                                 const QS_CONST R *m, const Ctx<I> &c,
                                                 const Regs<A, I> &g) {
@@ -739,7 +752,7 @@ constexpr int wave_round_regs() {
 // same op list, so they meet the same barriers.
 // KIND: 0 uncontrolled 2x2, 1 controlled 2x2, 2 (controlled) exchange,
 // 3 multiplexed 2x2 whose pattern has no register bit.
-template <int KIND, int A, typename R, typename I>
+template <int KIND, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_wave(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                         const QS_CONST R *m, const Ctx<I> &c,
                                         const Regs<A, I> &g) {
@@ -752,7 +765,7 @@ __device__ __forceinline__ void op_wave(Amp<R, 1 << A> &a, const QS_CONST OpRec 
   const bool hi = (w >> k) & 1;  // the same for the whole wave
   const int pw = w ^ (1 << k);
   Pred<I> pd{};
-  if constexpr (KIND != 0) pd = make_pred<A>(op, c, g);
+  if constexpr (KIND != 0) pd = make_pred<A, RP>(op, c, g);
   R sr = 0, si = 0, pr = 0, pi = 0;  // coefficients of this wave's own and partner amplitude
   if constexpr (KIND == 0 || KIND == 1) {
     R mv[8];
@@ -808,10 +821,10 @@ __device__ __forceinline__ void op_wave(Amp<R, 1 << A> &a, const QS_CONST OpRec 
   }
 }
 
-template <int J, int A, typename R, typename I>
+template <int J, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                           const Ctx<I> &c, const Regs<A, I> &g) {
-  const Pred<I> p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A, RP>(op, c, g);
   constexpr int bj = 1 << J;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
@@ -825,11 +838,11 @@ __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRe
   }
 }
 
-template <int K, int A, typename R, typename I>
+template <int K, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                            const Ctx<I> &c, const Regs<A, I> &g) {
   const bool hi = (c.lane >> K) & 1;
-  const Pred<I> p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A, RP>(op, c, g);
   if constexpr (K >= 4) {
     // (controlled) exchange by half swaps: each lane swaps its pair's two
     // amplitudes where the pattern holds (registers r, r | 1 paired)
@@ -858,10 +871,10 @@ __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
   }
 }
 
-template <int J0, int J1, int A, typename R, typename I>
+template <int J0, int J1, int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                           const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
-  const Pred<I> p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A, RP>(op, c, g);
   constexpr int b0 = 1 << J0, b1 = 1 << J1;
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
@@ -890,11 +903,11 @@ __device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRe
   }
 }
 
-template <int A, typename R, typename I>
+template <int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                           const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
   const R mr = m[0], mi = m[1];
-  const Pred<I> p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A, RP>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const bool on = pred_on<A>(p, g, r);
@@ -908,10 +921,10 @@ __device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const QS_CONST OpRe
 
 // Sign flip where the pattern holds: the high word of each value carries
 // the sign, so one select per amplitude and two integer xors.
-template <int A, typename R, typename I>
+template <int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_neg(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                        const Ctx<I> &c, const Regs<A, I> &g) {
-  const Pred<I> p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A, RP>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const uint32_t flip = pred_on<A>(p, g, r) ? 0x80000000u : 0u;
@@ -925,10 +938,10 @@ __device__ __forceinline__ void op_neg(Amp<R, 1 << A> &a, const QS_CONST OpRec &
   }
 }
 
-template <int A, typename R, typename I>
+template <int A, int RP, typename R, typename I>
 __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                         const QS_CONST R *m, const Ctx<I> &c, const Regs<A, I> &g) {
-  const Pred<I> p = make_pred<A>(op, c, g);
+  const Pred<I> p = make_pred<A, RP>(op, c, g);
   const int na = op.na;
   const int q0 = op.act[0], q1 = na > 1 ? op.act[1] : op.act[0];
   R dv[8];
@@ -957,43 +970,43 @@ __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec 
 // (type and register slots folded together, see op_code).  Codes naming a
 // register slot the kernel does not have (j >= A) or a wave bit of a
 // kernel without wave bits compile to nothing, like CODE_NONE.
-template <uint32_t CODE, int A, int W, typename R, typename I>
+template <uint32_t CODE, int A, int W, int RP = -1, typename R, typename I>
 __device__ __forceinline__ void apply_code(Amp<R, 1 << A> &a, const QS_CONST OpRec &op,
                                            const QS_CONST R *m, const Ctx<I> &c,
                                            const Regs<A, I> &g) {
   constexpr uint32_t C = CODE;
   if constexpr (C >= CODE_U1 && C < CODE_U1 + kMaxA) {
-    if constexpr (C - CODE_U1 < (uint32_t)A) op_1q_reg<C - CODE_U1, false, A>(a, op, m, c, g);
+    if constexpr (C - CODE_U1 < (uint32_t)A) op_1q_reg<C - CODE_U1, false, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_U1 + kMaxA && C < CODE_C1) {
-    op_1q_lane<false, C - CODE_U1 - kMaxA, A>(a, op, m, c, g);
+    op_1q_lane<false, C - CODE_U1 - kMaxA, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_C1 && C < CODE_C1 + kMaxA) {
-    if constexpr (C - CODE_C1 < (uint32_t)A) op_1q_reg<C - CODE_C1, true, A>(a, op, m, c, g);
+    if constexpr (C - CODE_C1 < (uint32_t)A) op_1q_reg<C - CODE_C1, true, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_C1 + kMaxA && C < CODE_X1) {
-    op_1q_lane<true, C - CODE_C1 - kMaxA, A>(a, op, m, c, g);
+    op_1q_lane<true, C - CODE_C1 - kMaxA, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_X1 && C < CODE_X1 + kMaxA) {
-    if constexpr (C - CODE_X1 < (uint32_t)A) op_x1_reg<C - CODE_X1, A, R, I>(a, op, c, g);
+    if constexpr (C - CODE_X1 < (uint32_t)A) op_x1_reg<C - CODE_X1, A, RP, R, I>(a, op, c, g);
   } else if constexpr (C >= CODE_X1 + kMaxA && C < CODE_SPARSE) {
-    op_x1_lane<C - CODE_X1 - kMaxA, A, R, I>(a, op, c, g);
+    op_x1_lane<C - CODE_X1 - kMaxA, A, RP, R, I>(a, op, c, g);
   } else if constexpr (C == CODE_SPARSE) {
-    op_sparse<A>(a, op, m, c, g);
+    op_sparse<A, RP>(a, op, m, c, g);
   } else if constexpr (C == CODE_DIAG) {
-    op_diag<A>(a, op, m, c, g);
+    op_diag<A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_DENSE2 && C < CODE_MUXL) {
     constexpr int J0 = (int)((C - CODE_DENSE2) / kMaxA), J1 = (int)((C - CODE_DENSE2) % kMaxA);
-    if constexpr (J0 < A && J1 < A && J0 != J1) op_dense2<J0, J1, A>(a, op, m, c, g);
+    if constexpr (J0 < A && J1 < A && J0 != J1) op_dense2<J0, J1, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_MUXL && C < CODE_MUXL + kMaxA) {
-    if constexpr (C - CODE_MUXL < (uint32_t)A) op_mux_reg<C - CODE_MUXL, false, A>(a, op, m, c, g);
+    if constexpr (C - CODE_MUXL < (uint32_t)A) op_mux_reg<C - CODE_MUXL, false, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_MUXL + kMaxA && C < CODE_MUXR) {
-    op_mux_lane<C - CODE_MUXL - kMaxA, A>(a, op, m, c, g);
+    op_mux_lane<C - CODE_MUXL - kMaxA, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_MUXR && C < CODE_MUXRL) {
-    if constexpr (C - CODE_MUXR < (uint32_t)A) op_mux_reg<C - CODE_MUXR, true, A>(a, op, m, c, g);
+    if constexpr (C - CODE_MUXR < (uint32_t)A) op_mux_reg<C - CODE_MUXR, true, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_MUXRL && C < CODE_U1W) {
     constexpr int JC = (int)((C - CODE_MUXRL) / kLaneBits), LK = (int)((C - CODE_MUXRL) % kLaneBits);
-    if constexpr (JC < A) op_mux_lane_reg<JC, LK, A>(a, op, m, c, g);
+    if constexpr (JC < A) op_mux_lane_reg<JC, LK, A, RP>(a, op, m, c, g);
   } else if constexpr (C >= CODE_U1W && C <= CODE_MUXW) {
-    if constexpr (W > 0) op_wave<(int)(C - CODE_U1W), A>(a, op, m, c, g);
+    if constexpr (W > 0) op_wave<(int)(C - CODE_U1W), A, RP>(a, op, m, c, g);
   } else if constexpr (C == CODE_NEG) {
-    op_neg<A, R, I>(a, op, c, g);
+    op_neg<A, RP, R, I>(a, op, c, g);
   }
 }
 
