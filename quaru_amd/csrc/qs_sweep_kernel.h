@@ -230,41 +230,47 @@ struct Ctx {
   int wave;  // wave index in the workgroup
 };
 
-// on(r) = lane_ok && ((r & rm) == rv): the second factor is uniform.
+// on(r) = lane_ok && ((goff(r) & cmr) == cvr): the second factor is uniform
+// (in a specialised kernel a constant, see make_pred).
 template <typename I>
 struct Pred {
   bool lane_ok;
-  uint32_t rm, rv;  // the pattern on the register slots: register r meets it if (r & rm) == rv
+  I cmr, cvr;
 };
 
-// RP >= 0: the register-slot pattern known when the kernel was generated
-// (rm in bits 0..7, rv in bits 8..15), so that the per-register tests below
-// fold away; RP < 0: taken from the op's pattern and the slots' qubits.
+// RP >= 0: the op's pattern on the register slots known when the kernel
+// was generated, in slot space (register r meets it if (r & rm) == rv, rm in
+// bits 0..7 of RP, rv in bits 8..15), so that the per-register tests fold
+// away; RP < 0: the pattern's global bits, compared with each register's
+// offset at run time.
 template <int A, int RP, typename I>
 __device__ __forceinline__ Pred<I> make_pred(const QS_CONST OpRec &op, const Ctx<I> &c,
                                              const Regs<A, I> &g) {
   const I cm = (I)op.cmask, cv = (I)op.cval;
   const I cmo = cm & ~g.mask;
   Pred<I> p;
-  if constexpr (RP >= 0) {
-    p.rm = (uint32_t)RP & 0xFFu;
-    p.rv = ((uint32_t)RP >> 8) & 0xFFu;
+  if constexpr (RP < 0) {
+    p.cmr = cm & g.mask;
+    p.cvr = cv & g.mask;
   } else {
-    p.rm = 0;
-    p.rv = 0;
-#pragma unroll
-    for (int j = 0; j < A; ++j) {
-      if (cm & g.gb[j]) p.rm |= 1u << j;
-      if (cv & g.gb[j]) p.rv |= 1u << j;
-    }
+    p.cmr = 0;
+    p.cvr = 0;
   }
   p.lane_ok = (c.gbase & cmo) == (cv & cmo);
   return p;
 }
 
-template <int A, typename I>
-__device__ __forceinline__ bool pred_on(const Pred<I> &p, const Regs<A, I> &, int r) {
-  return p.lane_ok && (((uint32_t)r & p.rm) == p.rv);
+template <int A, int RP, typename I>
+__device__ __forceinline__ bool reg_on(const Pred<I> &p, const Regs<A, I> &g, int r) {
+  if constexpr (RP >= 0)
+    return ((uint32_t)r & ((uint32_t)RP & 0xFFu)) == (((uint32_t)RP >> 8) & 0xFFu);
+  else
+    return (goff<A>(g, r) & p.cmr) == p.cvr;
+}
+
+template <int A, int RP, typename I>
+__device__ __forceinline__ bool pred_on(const Pred<I> &p, const Regs<A, I> &g, int r) {
+  return p.lane_ok && reg_on<A, RP>(p, g, r);
 }
 
 // Partner lane's value (lane ^ 2^K) by VALU cross-lane instructions: DPP
@@ -446,7 +452,7 @@ __device__ __forceinline__ void op_1q_reg(Amp<R, 1 << A> &a, const QS_CONST OpRe
     cfma(wr, wi, m2r, m2i, xr, xi);
     cfma(wr, wi, m3r, m3i, yr, yi);
     if constexpr (kCtl) {
-      const bool on = pred_on<A>(p, g, r);
+      const bool on = pred_on<A, RP>(p, g, r);
       a.re[r] = on ? ur : xr;
       a.im[r] = on ? ui : xi;
       a.re[r | bj] = on ? wr : yr;
@@ -512,7 +518,7 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
           for (int q = 0; q < 8; ++q) t[q] = m[q];
         },
         [&](int r, bool h) {
-          if constexpr (kCtl) return p.lane_ok && (h ? pred_on<A>(p, g, r | 1) : pred_on<A>(p, g, r));
+          if constexpr (kCtl) return p.lane_ok && (h ? pred_on<A, RP>(p, g, r | 1) : pred_on<A, RP>(p, g, r));
           return true;
         });
     return;
@@ -537,7 +543,7 @@ __device__ __forceinline__ void op_1q_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
     cfma(ur, ui, sr, si, vr, vi);
     cfma(ur, ui, pr, pi, yr, yi);
     if constexpr (kCtl) {
-      const bool on = pred_on<A>(p, g, r);
+      const bool on = pred_on<A, RP>(p, g, r);
       a.re[r] = on ? ur : xr;
       a.im[r] = on ? ui : xi;
     } else {
@@ -595,7 +601,7 @@ __device__ __forceinline__ void op_mux_reg(Amp<R, 1 << A> &a, const QS_CONST OpR
     if ((r >> J) & 1) continue;
     R tr[4], ti[4];
     if constexpr (kRegPattern) {
-      const bool hold = ((uint32_t)r & p.rm) == p.rv;  // the same for the whole wave
+      const bool hold = reg_on<A, RP>(p, g, r);  // the same for the whole wave
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         tr[k] = hold ? v1r[k] : v0r[k];
@@ -801,7 +807,7 @@ __device__ __forceinline__ void op_wave(Amp<R, 1 << A> &a, const QS_CONST OpRec 
       const R yi = reg[((pw * NR + j) * 2 + 1) * kWave + c.lane];
       const R xr = a.re[r], xi = a.im[r];
       if constexpr (KIND == 2) {
-        const bool on = pred_on<A>(pd, g, r);
+        const bool on = pred_on<A, RP>(pd, g, r);
         a.re[r] = on ? yr : xr;
         a.im[r] = on ? yi : xi;
       } else {
@@ -809,7 +815,7 @@ __device__ __forceinline__ void op_wave(Amp<R, 1 << A> &a, const QS_CONST OpRec 
         cfma(ur, ui, sr, si, xr, xi);
         cfma(ur, ui, pr, pi, yr, yi);
         if constexpr (KIND == 1) {
-          const bool on = pred_on<A>(pd, g, r);
+          const bool on = pred_on<A, RP>(pd, g, r);
           a.re[r] = on ? ur : xr;
           a.im[r] = on ? ui : xi;
         } else {
@@ -830,7 +836,7 @@ __device__ __forceinline__ void op_x1_reg(Amp<R, 1 << A> &a, const QS_CONST OpRe
   for (int r = 0; r < (1 << A); ++r) {
     if ((r >> J) & 1) continue;
     const R xr = a.re[r], xi = a.im[r], yr = a.re[r | bj], yi = a.im[r | bj];
-    const bool on = pred_on<A>(p, g, r);
+    const bool on = pred_on<A, RP>(p, g, r);
     a.re[r] = on ? yr : xr;
     a.im[r] = on ? yi : xi;
     a.re[r | bj] = on ? xr : yr;
@@ -851,7 +857,7 @@ __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
       R ur = a.re[r], ui = a.im[r], wr = a.re[r | 1], wi = a.im[r | 1];
       half_swap<K>(ur, wr);
       half_swap<K>(ui, wi);
-      const bool on = p.lane_ok && (hi ? pred_on<A>(p, g, r | 1) : pred_on<A>(p, g, r));
+      const bool on = p.lane_ok && (hi ? pred_on<A, RP>(p, g, r | 1) : pred_on<A, RP>(p, g, r));
       R xr = on ? wr : ur, xi = on ? wi : ui, yr = on ? ur : wr, yi = on ? ui : wi;
       half_swap<K>(xr, yr);
       half_swap<K>(xi, yi);
@@ -865,7 +871,7 @@ __device__ __forceinline__ void op_x1_lane(Amp<R, 1 << A> &a, const QS_CONST OpR
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
     const R yr = lane_xor<K>(a.re[r], hi), yi = lane_xor<K>(a.im[r], hi);
-    const bool on = pred_on<A>(p, g, r);
+    const bool on = pred_on<A, RP>(p, g, r);
     a.re[r] = on ? yr : a.re[r];
     a.im[r] = on ? yi : a.im[r];
   }
@@ -891,7 +897,7 @@ __device__ __forceinline__ void op_dense2(Amp<R, 1 << A> &a, const QS_CONST OpRe
       yr[i] = sr;
       yi[i] = si;
     }
-    const bool on = pred_on<A>(p, g, r);
+    const bool on = pred_on<A, RP>(p, g, r);
     a.re[i0] = on ? yr[0] : xr[0];
     a.im[i0] = on ? yi[0] : xi[0];
     a.re[i1] = on ? yr[1] : xr[1];
@@ -910,7 +916,7 @@ __device__ __forceinline__ void op_sparse(Amp<R, 1 << A> &a, const QS_CONST OpRe
   const Pred<I> p = make_pred<A, RP>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
-    const bool on = pred_on<A>(p, g, r);
+    const bool on = pred_on<A, RP>(p, g, r);
     const R xr = a.re[r], xi = a.im[r];
     R sr = 0, si = 0;
     cfma(sr, si, mr, mi, xr, xi);
@@ -927,7 +933,7 @@ __device__ __forceinline__ void op_neg(Amp<R, 1 << A> &a, const QS_CONST OpRec &
   const Pred<I> p = make_pred<A, RP>(op, c, g);
 #pragma unroll
   for (int r = 0; r < (1 << A); ++r) {
-    const uint32_t flip = pred_on<A>(p, g, r) ? 0x80000000u : 0u;
+    const uint32_t flip = pred_on<A, RP>(p, g, r) ? 0x80000000u : 0u;
     if constexpr (sizeof(R) == 8) {
       a.re[r] = __hiloint2double(__double2hiint(a.re[r]) ^ (int)flip, __double2loint(a.re[r]));
       a.im[r] = __hiloint2double(__double2hiint(a.im[r]) ^ (int)flip, __double2loint(a.im[r]));
@@ -954,7 +960,7 @@ __device__ __forceinline__ void op_diag(Amp<R, 1 << A> &a, const QS_CONST OpRec 
     const bool b1 = na > 1 && ((gi >> q1) & 1u);
     const R vr = b1 ? (b0 ? dv[6] : dv[4]) : (b0 ? dv[2] : dv[0]);
     const R vi = b1 ? (b0 ? dv[7] : dv[5]) : (b0 ? dv[3] : dv[1]);
-    const bool on = pred_on<A>(p, g, r);
+    const bool on = pred_on<A, RP>(p, g, r);
     const R xr = a.re[r], xi = a.im[r];
     R sr = 0, si = 0;
     cfma(sr, si, vr, vi, xr, xi);
