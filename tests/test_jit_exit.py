@@ -60,3 +60,17 @@ def test_exit_status_kept(prog):
         assert p.returncode == int(status), f"exit status {p.returncode}\n{p.stdout}{p.stderr}"
         assert "main returns" in p.stdout
         assert dt < 10.0
+
+
+def test_register_patterns_match_the_run_time_test(tmp_path):
+    """The register-slot patterns the specialised kernels take as constants
+    (slot mask and values) select exactly the registers the generic kernel's
+    run-time test selects, for random slot qubits and patterns."""
+    out = str(tmp_path / "jit_patterns")
+    src = os.path.join(ROOT, "tests", "cpp", "jit_patterns.cpp")
+    subprocess.run([HIPCC, "-std=c++17", "-O1", "-I", CSRC, src, "-L", LIBDIR, "-lquaru_hip",
+                    f"-Wl,-rpath,{LIBDIR}", "-o", out],
+                   check=True, capture_output=True, timeout=600)
+    p = subprocess.run([out], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.startswith("checked "), p.stdout
