@@ -275,38 +275,65 @@ __device__ __forceinline__ float lane_partner(float v) {
 // its partner's amplitude by DPP, and computes its own output with row
 // ((l >> K) & 1) of the matrix.  Chunk c of 64 amplitudes is expand(c, fx)
 // << 6 with fx holding the controls relative to bit 6.
-template <typename R, int K, int kItems, bool kNT>
+//
+// Chunk order (kStreams == 0): wave w takes chunks w * kItems + it.  With
+// kStreams = S = 2^SB (nchunks a multiple of 2^(5 + SB)), a workgroup reads
+// S streams 32 chunks apart: unit u = block * 4 * kItems + it * 4 + wave
+// gives chunks (u mod 32) | (u / 32) << (5 + SB) | s << 5 for s < S.
+// Measured at 30 qubits, complex128, gate on qubit 0: S = 4, kItems = 4
+// (16 loads in flight per lane, 64 KiB per workgroup in four 16 KiB runs)
+// 5.28 ms; one stream of 4 chunks per wave 5.92 ms; S = 2 5.55-5.70 ms;
+// S = 8 5.42-5.96 ms; streams 16, 64 or 128 chunks apart 5.38-5.93 ms.
+template <typename R, int K, int kItems, bool kNT, int kStreams = 0>
 __global__ __launch_bounds__(kBlock) void k_gate1_lane(typename V2<R>::T *__restrict__ psi,
                                                        uint64_t nchunks, Fixed fx,
                                                        MatArg<R, 2> m) {
   using T = typename V2<R>::T;
+  constexpr int S = kStreams > 0 ? kStreams : 1;
+  constexpr int SB = S == 1 ? 0 : (S == 2 ? 1 : (S == 4 ? 2 : 3));
+  static_assert(S == (1 << SB), "kStreams: 0, 1, 2, 4 or 8");
+  constexpr int NL = kItems * S;
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const bool hi = (lane >> K) & 1;
   // this lane's row: coefficient of its own amplitude (s) and the partner's (p)
   const R sr = hi ? m.re[3] : m.re[0], si = hi ? m.im[3] : m.im[0];
   const R pr = hi ? m.re[2] : m.re[1], pi = hi ? m.im[2] : m.im[1];
-  T a[kItems];
-  uint64_t idx[kItems];
+  T a[NL];
+  uint64_t idx[NL];
+  bool ok[NL];
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
-    const uint64_t c = wave * kItems + it;
-    if (c < nchunks) {
-      idx[it] = (expand(c, fx) << 6) | (uint64_t)lane;
-      a[it] = ld<kNT>(psi + idx[it]);
+    if constexpr (kStreams == 0) {
+      const uint64_t c = wave * kItems + it;
+      ok[it] = c < nchunks;  // uniform over the wave: whole chunks only
+      if (ok[it]) idx[it] = (expand(c, fx) << 6) | (uint64_t)lane;
+    } else {
+      // the workgroup's 4 * kItems * S chunks lie below nchunks as a whole or
+      // not at all (the host launches nchunks / (4 * kItems * S) workgroups)
+      const bool in = ((uint64_t)blockIdx.x + 1) * (4 * kItems * S) <= nchunks;
+      const uint64_t u = (uint64_t)blockIdx.x * (4 * kItems) + it * 4 + (threadIdx.x >> 6);
+      const uint64_t c0 = (u & 31ull) | ((u >> 5) << (5 + SB));
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        ok[it * S + s] = in;
+        if (in) idx[it * S + s] = (expand(c0 | ((uint64_t)s << 5), fx) << 6) | (uint64_t)lane;
+      }
     }
   }
 #pragma unroll
-  for (int it = 0; it < kItems; ++it) {
-    const uint64_t c = wave * kItems + it;
-    if (c < nchunks) {  // uniform over the wave: whole chunks only
+  for (int j = 0; j < NL; ++j)
+    if (ok[j]) a[j] = ld<kNT>(psi + idx[j]);
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    if (ok[j]) {
       T y;
-      y.x = lane_partner<K>(a[it].x);
-      y.y = lane_partner<K>(a[it].y);
+      y.x = lane_partner<K>(a[j].x);
+      y.y = lane_partner<K>(a[j].y);
       T r = czero<T>();
-      r = cmla(r, sr, si, a[it]);
+      r = cmla(r, sr, si, a[j]);
       r = cmla(r, pr, pi, y);
-      st<kNT>(psi + idx[it], r);
+      st<kNT>(psi + idx[j], r);
     }
   }
 }
@@ -777,19 +804,27 @@ hipError_t launch_gate_dense(void *psi_v, int n, const Gate &g, void *d_mat, voi
     return !(v && v[0] == '0');
   }();
   if (g.na == 1 && g.act[0] < 6 && n >= 6 && (g.cmask & 0x3Full) == 0 && lane_form) {
-    // measured at 30 qubits, complex128: 5.85 ms per gate on qubits 0..5
-    // against 6.28-6.54 ms for k_gate1; 8 chunks per wave, or two runs 64 KiB
-    // apart per workgroup, ran slower (6.0-6.4 ms)
+    // measured at 30 qubits, complex128, on qubit 0: four streams 32 chunks
+    // apart per workgroup 5.28 ms, one run of 4 chunks per wave 5.92 ms,
+    // k_gate1 6.28-6.54 ms.  States of fewer than 128 chunks take the one-run
+    // order (the four-stream order needs chunk bits 5 and 6).
     constexpr int kItems = 4;
+    constexpr int kStreams = 4;
     const Fixed fxc = make_fixed(g.cmask >> 6, g.cval >> 6);
     const uint64_t nchunks = 1ull << (n - 6 - fxc.nf);
-    const unsigned grid = grid_for(nchunks * 64, kBlock * kItems);
+    const bool streams = nchunks >= 128;
+    const unsigned grid = streams ? (unsigned)(nchunks / (4 * kItems * kStreams))
+                                  : grid_for(nchunks * 64, kBlock * kItems);
     const MatArg<R, 2> ma = to_matarg<R, 2>(g);
     nt_dispatch(big, [&](auto nt) {
       constexpr bool NT = decltype(nt)::value;
       auto go = [&](auto kk) {
-        hipLaunchKernelGGL((k_gate1_lane<R, decltype(kk)::value, kItems, NT>), dim3(grid),
-                           dim3(kBlock), 0, s, psi, nchunks, fxc, ma);
+        if (streams)
+          hipLaunchKernelGGL((k_gate1_lane<R, decltype(kk)::value, kItems, NT, kStreams>),
+                             dim3(grid), dim3(kBlock), 0, s, psi, nchunks, fxc, ma);
+        else
+          hipLaunchKernelGGL((k_gate1_lane<R, decltype(kk)::value, kItems, NT>), dim3(grid),
+                             dim3(kBlock), 0, s, psi, nchunks, fxc, ma);
       };
       switch (g.act[0]) {
         case 0: go(std::integral_constant<int, 0>{}); break;

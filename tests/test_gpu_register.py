@@ -212,6 +212,32 @@ def test_gate_zoo_parity_c128(n, fusion):
     assert maxdiff(r.state, ref) < TOL
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [12, 13, 14, 15])
+def test_one_qubit_lane_form_chunk_orders(n, dtype):
+    """Fusion off, one-qubit gates on qubits 0..5 with no control there: the
+    lane kernel takes its four-stream chunk order from 128 chunks of 64
+    amplitudes on (13 qubits uncontrolled, 14 with one control on a bit >= 6,
+    15 with two) and its one-run order below; both against the oracle."""
+    rng = np.random.default_rng(300 + n)
+    psi = random_state(rng, n)
+    r = Register.new([False] * n, dtype).set_fusion(False)
+    r.state = psi
+    opl = [ops.Operation(random_unitary(rng, 2), [q]) for q in range(6)]
+    opl += [ops.to_controlled(ops.Operation(random_unitary(rng, 2), [q]), n - 1) for q in (0, 3, 5)]
+    opl += [ops.Operation.controlled([6, n - 2], random_unitary(rng, 2), [q],
+                                     control_values=[True, False]) for q in (1, 4)]
+    ref = psi
+    for op in opl:
+        r.apply(op)
+        ref = oracle.apply_fast(ref, op.matrix(), op.targets())
+    got = r.state
+    if dtype == "c128":
+        assert maxdiff(got, ref) < TOL
+    else:
+        assert maxdiff(got, ref) < 1e-5
+
+
 @pytest.mark.parametrize("n", [12, 14])
 def test_gate_zoo_parity_c64(n):
     rng = np.random.default_rng(200 + n)
