@@ -195,36 +195,50 @@ struct MatArg {
 };
 
 // One active target (matrix bit 0 <-> qubit with stride `tbit`).  Each thread
-// owns kItems pairs spaced one block apart, so a wave issues all its loads
-// before the first multiply.
-template <typename R, int kItems, bool kNT>
+// owns kItems units spaced one block apart, so a wave issues all its loads
+// before the first multiply.  kX = 1: a unit is one pair (i, i | tbit).
+// kX = 2: a unit is the two pairs at i and i | xbit, xbit a free index bit
+// that fx also holds at 0, so a workgroup reads four streams (xbit 32 or 64
+// KiB of amplitudes, as the host picks it; complex128 only).  Measured at 30 qubits,
+// complex128, Haar gate, ms per launch: target 6-10 6.25-6.28 with kX = 1,
+// 5.61-5.69 with kX = 2 and xbit = 2^12; target 13-29 6.00-6.10 against
+// 5.37-5.53; target 11 5.27 against 5.22.  An xbit of 2^13 ran 6.0-6.9.
+template <typename R, int kItems, bool kNT, int kX = 1>
 __global__ __launch_bounds__(kBlock) void k_gate1(typename V2<R>::T *__restrict__ psi,
                                                   uint64_t npairs, Fixed fx, uint64_t tbit,
-                                                  MatArg<R, 2> m) {
+                                                  MatArg<R, 2> m, uint64_t xbit = 0) {
   using T = typename V2<R>::T;
+  static_assert(kX == 1 || kX == 2, "kX: 1 or 2 pairs per unit");
+  constexpr int NP = kItems * kX;
   const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kItems) + threadIdx.x;
-  T a0[kItems], a1[kItems];
-  uint64_t i0[kItems];
+  T a0[NP], a1[NP];
+  uint64_t i0[NP];
+  bool ok[kItems];
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const uint64_t p = base + (uint64_t)it * kBlock;
-    if (p < npairs) {
-      i0[it] = expand(p, fx);
-      a0[it] = ld<kNT>(psi + i0[it]);
-      a1[it] = ld<kNT>(psi + (i0[it] | tbit));
+    ok[it] = p < npairs;
+    if (ok[it]) {
+      const uint64_t i = expand(p, fx);
+#pragma unroll
+      for (int x = 0; x < kX; ++x) {
+        const int j = it * kX + x;
+        i0[j] = x ? (i | xbit) : i;
+        a0[j] = ld<kNT>(psi + i0[j]);
+        a1[j] = ld<kNT>(psi + (i0[j] | tbit));
+      }
     }
   }
 #pragma unroll
-  for (int it = 0; it < kItems; ++it) {
-    const uint64_t p = base + (uint64_t)it * kBlock;
-    if (p < npairs) {
+  for (int j = 0; j < NP; ++j) {
+    if (ok[j / kX]) {
       T r0 = czero<T>(), r1 = czero<T>();
-      r0 = cmla(r0, m.re[0], m.im[0], a0[it]);
-      r0 = cmla(r0, m.re[1], m.im[1], a1[it]);
-      r1 = cmla(r1, m.re[2], m.im[2], a0[it]);
-      r1 = cmla(r1, m.re[3], m.im[3], a1[it]);
-      st<kNT>(psi + i0[it], r0);
-      st<kNT>(psi + (i0[it] | tbit), r1);
+      r0 = cmla(r0, m.re[0], m.im[0], a0[j]);
+      r0 = cmla(r0, m.re[1], m.im[1], a1[j]);
+      r1 = cmla(r1, m.re[2], m.im[2], a0[j]);
+      r1 = cmla(r1, m.re[3], m.im[3], a1[j]);
+      st<kNT>(psi + i0[j], r0);
+      st<kNT>(psi + (i0[j] | tbit), r1);
     }
   }
 }
@@ -837,11 +851,35 @@ hipError_t launch_gate_dense(void *psi_v, int n, const Gate &g, void *d_mat, voi
     });
   } else if (g.na == 1) {
     constexpr int kItems = 4;
-    nt_dispatch(big && !near(g.act[0]), [&](auto nt) {
-      hipLaunchKernelGGL((k_gate1<R, kItems, decltype(nt)::value>),
-                         dim3(grid_for(nfreeN, kBlock * kItems)), dim3(kBlock), 0, s, psi, nfreeN,
-                         fx, 1ull << g.act[0], to_matarg<R, 2>(g));
-    });
+    // streaming complex128 states, target not on qubits 0..2: two pairs per
+    // unit, the second xbit away; xbit is qubit 12 (64 KiB of amplitudes)
+    // when free, else qubit 11.  Complex64 keeps one pair per unit: at 30
+    // qubits the two forms measured within +-4% of each other, either way
+    // depending on the target (wall time per gate, 2.57-3.01 ms).
+    // QS_GATE1_STREAMS=0 keeps one pair per unit.
+    static const bool streams_on = [] {
+      const char *v = std::getenv("QS_GATE1_STREAMS");
+      return !(v && v[0] == '0');
+    }();
+    const uint64_t used = g.cmask | amask;
+    int xq = -1;
+    if (sizeof(T) == 16 && big && !near(g.act[0]) && streams_on) {
+      for (int q : {12, 11})
+        if (xq < 0 && q < n && !((used >> q) & 1ull)) xq = q;
+    }
+    if (xq >= 0) {
+      const Fixed fxx = make_fixed(used | (1ull << xq), g.cval);
+      const uint64_t nunits = nfreeN >> 1;
+      hipLaunchKernelGGL((k_gate1<R, kItems, true, 2>), dim3(grid_for(nunits, kBlock * kItems)),
+                         dim3(kBlock), 0, s, psi, nunits, fxx, 1ull << g.act[0],
+                         to_matarg<R, 2>(g), 1ull << xq);
+    } else {
+      nt_dispatch(big && !near(g.act[0]), [&](auto nt) {
+        hipLaunchKernelGGL((k_gate1<R, kItems, decltype(nt)::value>),
+                           dim3(grid_for(nfreeN, kBlock * kItems)), dim3(kBlock), 0, s, psi,
+                           nfreeN, fx, 1ull << g.act[0], to_matarg<R, 2>(g), (uint64_t)0);
+      });
+    }
   } else if (g.na == 2) {
     constexpr int kItems = 2;
     nt_dispatch(big && !near(g.act[0]) && !near(g.act[1]), [&](auto nt) {

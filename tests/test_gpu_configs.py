@@ -128,6 +128,11 @@ def _streaming_zoo(rng, n):
     opl += [ops.to_controlled(ops.Operation(random_unitary(rng, 2), [1]), 20),
             ops.Operation.controlled([7, 18], random_unitary(rng, 2), [2],
                                      control_values=[True, False])]
+    # complex128 one-qubit gates with a control on qubit 12 (the second pair
+    # of a unit moves to qubit 11) and on 11..13 (no free qubit left for it:
+    # one pair per unit)
+    opl += [ops.to_controlled(ops.Operation(random_unitary(rng, 2), [20]), 12),
+            ops.Operation.controlled([11, 12, 13], random_unitary(rng, 2), [21])]
     opl += [ops.cnot(0, n - 1), ops.cnx([3, 9], 17), ops.cnz([2, 5, 11], 23), ops.rz(0.7, 14),
             ops.Operation(np.diag(np.exp(1j * rng.uniform(0, 6, 8))), [4, 13, 22]),
             ops.to_controlled(ops.Operation(random_unitary(rng, 2), [n - 3]), 1)]
