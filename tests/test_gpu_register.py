@@ -438,15 +438,17 @@ def test_measure_matches_oracle(dtype):
             assert maxdiff(r.state, ref) < tol
 
 
+@pytest.mark.parametrize("n", [13, 22])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-def test_measure_chain_with_pending_collapses(dtype):
+def test_measure_chain_with_pending_collapses(dtype, n):
     """Measures one after another (each collapse runs fused with the next
     measure's probability split) mixed with gates, readouts, a clone and a
     full readout, each of which must first see the collapse applied: the
     outcomes equal the oracle's on the same uniforms, the state after every
-    step too (src/register.rs:291-342)."""
+    step too (src/register.rs:291-342).  At 22 qubits the split, the fused
+    collapse and split and the norm take their block order (each workgroup a
+    contiguous range); at 13 the grid-stride order."""
     rng = np.random.default_rng(29)
-    n = 13
     psi = random_state(rng, n)
     tol = TOL if dtype == "c128" else 1e-6
     r = Register.new([False] * n, dtype)
@@ -480,6 +482,9 @@ def test_measure_chain_with_pending_collapses(dtype):
     ridx, ref = oracle.measure_sequence(ref, [2, 4, 6], u)
     assert idx == ridx
     meas(11, 0.5)
+    meas(n - 1, 0.4)
+    meas(n - 2, 0.7)         # fused with the collapse on qubit n - 1
+    assert abs(r.norm_sqr() - float(np.vdot(ref, ref).real)) < (1e-9 if dtype == "c128" else 1e-6)
     assert maxdiff(r.state, ref) < tol
     r.close()
 
