@@ -438,16 +438,17 @@ def test_measure_matches_oracle(dtype):
             assert maxdiff(r.state, ref) < tol
 
 
-@pytest.mark.parametrize("n", [13, 22])
+@pytest.mark.parametrize("n", [13, 22, 25])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 def test_measure_chain_with_pending_collapses(dtype, n):
     """Measures one after another (each collapse runs fused with the next
     measure's probability split) mixed with gates, readouts, a clone and a
     full readout, each of which must first see the collapse applied: the
     outcomes equal the oracle's on the same uniforms, the state after every
-    step too (src/register.rs:291-342).  At 22 qubits the split, the fused
-    collapse and split and the norm take their block order (each workgroup a
-    contiguous range); at 13 the grid-stride order."""
+    step too (src/register.rs:291-342).  From 22 qubits on the split, the
+    fused collapse and split and the norm take their block order (each
+    workgroup a contiguous range), at 13 the grid-stride order; 25 qubits
+    (512 MiB complex128) is a state beyond the last-level cache."""
     rng = np.random.default_rng(29)
     psi = random_state(rng, n)
     tol = TOL if dtype == "c128" else 1e-6
@@ -502,7 +503,7 @@ def test_measure_basis_and_bell():
         assert r.measure(i) == r.measure(j)
 
 
-@pytest.mark.parametrize("n", [10, 14, 20])
+@pytest.mark.parametrize("n", [10, 14, 20, 25])
 def test_measure_sequence_matches_oracle(n):
     rng = np.random.default_rng(n)
     psi = random_state(rng, n)
