@@ -1609,15 +1609,23 @@ __global__ __launch_bounds__(kBlock) void k_set_hist(const typename V2<R>::T *__
   const int H = 1 << m;
   const int S = H >= kBlock ? 1 : kBlock / H;
   const int phase = H >= kBlock ? 0 : (int)threadIdx.x / H;
-  uint64_t dep[BPT];
+  // bin h's offset is the deposit of its bits at po.p: the thread's part
+  // (bits 0..7 of h, h < 256: one value per thread) ORed with bin k's part
+  // (bits 8.. of t + 256 k: the same for every thread, kept on scalar
+  // registers), so that a thread holds one 64-bit offset, not BPT
+  const int lo_bits = m < 8 ? m : 8;
+  const int tl = H >= kBlock ? (int)threadIdx.x : (int)threadIdx.x % H;
+  uint64_t dlo = 0;
+  for (int b = 0; b < lo_bits; ++b)
+    if ((tl >> b) & 1) dlo |= 1ull << po.p[b];
+  uint64_t dhi[BPT];
   double acc[BPT];
 #pragma unroll
   for (int k = 0; k < BPT; ++k) {
-    const int h = H >= kBlock ? (int)threadIdx.x + k * kBlock : (int)threadIdx.x % H;
     uint64_t d = 0;
-    for (int b = 0; b < m; ++b)
-      if ((h >> b) & 1) d |= 1ull << po.p[b];
-    dep[k] = d;
+    for (int b = lo_bits; b < m; ++b)
+      if (((k * kBlock) >> b) & 1) d |= 1ull << po.p[b];
+    dhi[k] = d;
     acc[k] = 0.0;
   }
   const uint64_t j0 = (uint64_t)blockIdx.x * per_block;
@@ -1629,7 +1637,7 @@ __global__ __launch_bounds__(kBlock) void k_set_hist(const typename V2<R>::T *__
     const uint64_t e = expand(j, fx);
     typename V2<R>::T a[BPT];
 #pragma unroll
-    for (int k = 0; k < BPT; ++k) a[k] = psi[e | dep[k]];
+    for (int k = 0; k < BPT; ++k) a[k] = psi[(e | dhi[k]) | dlo];
 #pragma unroll
     for (int k = 0; k < BPT; ++k)
       acc[k] += (double)a[k].x * (double)a[k].x + (double)a[k].y * (double)a[k].y;
