@@ -1377,9 +1377,52 @@ hipError_t launch_measure_deferred(void *psi, int n, int target, double x, doubl
   return hipGetLastError();
 }
 
+// k_collapse over chunks of 64 amplitudes in the stream order of
+// k_gate1_lane (four streams 32 chunks apart per workgroup, 16 chunks per
+// workgroup per stream); lane l takes amplitude l of each chunk, reads it
+// only when it is kept.  nchunks is a power of two >= 128; the host launches
+// nchunks / 64 workgroups.  Measured at 30 qubits, complex128, collapse on
+// qubit 29 (half the state read, all of it written): 4.08 ms (6.3 TB/s)
+// against 4.78 ms for the grid-stride k_collapse.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_collapse_streams(typename V2<R>::T *__restrict__ psi,
+                                                             uint64_t nchunks, int target,
+                                                             int keep, double sq) {
+  using T = typename V2<R>::T;
+  constexpr int kItems = 4, S = 4;
+  if (((uint64_t)blockIdx.x + 1) * (4 * kItems * S) > nchunks) return;
+  const uint64_t lane = threadIdx.x & 63;
+  uint64_t idx[kItems * S];
+  T a[kItems * S];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t u = (uint64_t)blockIdx.x * (4 * kItems) + it * 4 + (threadIdx.x >> 6);
+    const uint64_t c0 = (u & 31ull) | ((u >> 5) << 7);
+#pragma unroll
+    for (int st_ = 0; st_ < S; ++st_) idx[it * S + st_] = ((c0 | ((uint64_t)st_ << 5)) << 6) | lane;
+  }
+#pragma unroll
+  for (int j = 0; j < kItems * S; ++j)
+    a[j] = (int)((idx[j] >> target) & 1ull) == keep ? ld<true>(psi + idx[j]) : czero<T>();
+#pragma unroll
+  for (int j = 0; j < kItems * S; ++j) {
+    T r = a[j];
+    r.x = (R)((double)r.x / sq);
+    r.y = (R)((double)r.y / sq);
+    st<true>(psi + idx[j], r);
+  }
+}
+
 template <typename R>
 hipError_t launch_collapse(void *psi, int n, int target, int keep, double total, hipStream_t s) {
   const uint64_t len = 1ull << n;
+  if (streaming(n, sizeof(typename V2<R>::T))) {
+    // states above the last-level cache: the stream order
+    const uint64_t nchunks = len >> 6;
+    hipLaunchKernelGGL(k_collapse_streams<R>, dim3((unsigned)(nchunks / 64)), dim3(kBlock), 0, s,
+                       (typename V2<R>::T *)psi, nchunks, target, keep, sqrt(total));
+    return hipGetLastError();
+  }
   unsigned grid = grid_for(len, kBlock * 4);
   if (grid > 262144) grid = 262144;
   hipLaunchKernelGGL(k_collapse<R>, dim3(grid), dim3(kBlock), 0, s, (typename V2<R>::T *)psi, len,
