@@ -100,6 +100,12 @@ hipError_t launch_prob_split(const void *psi, int n, int target, double *d_part,
                              hipStream_t s);
 template <typename R>
 hipError_t launch_collapse(void *psi, int n, int target, int keep, double total, hipStream_t s);
+// Keep the amplitudes i with (i & mask) == pat, divided by sqrt(total); zero
+// the rest.  Stream order, for states of at least 2^13 amplitudes
+// (hipErrorInvalidValue below).
+template <typename R>
+hipError_t launch_collapse_pattern(void *psi, int n, uint64_t mask, uint64_t pat, double total,
+                                   hipStream_t s);
 // launch_collapse(ctarget, keep, total) and launch_prob_split(target) in one
 // pass, with the same {prob_0, prob_1} bits as the two in sequence.
 template <typename R>

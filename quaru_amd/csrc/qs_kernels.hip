@@ -1377,17 +1377,19 @@ hipError_t launch_measure_deferred(void *psi, int n, int target, double x, doubl
   return hipGetLastError();
 }
 
-// k_collapse over chunks of 64 amplitudes in the stream order of
-// k_gate1_lane (four streams 32 chunks apart per workgroup, 16 chunks per
-// workgroup per stream); lane l takes amplitude l of each chunk, reads it
-// only when it is kept.  nchunks is a power of two >= 128; the host launches
+// The collapse onto the amplitudes i with (i & mask) == pat (k_collapse:
+// one target bit; the mask collapse of measure_sequence: every decided
+// qubit), over chunks of 64 amplitudes in the stream order of k_gate1_lane
+// (four streams 32 chunks apart per workgroup, 16 chunks per workgroup per
+// stream); lane l takes amplitude l of each chunk, reads it only when it is
+// kept.  nchunks is a power of two >= 128; the host launches
 // nchunks / 64 workgroups.  Measured at 30 qubits, complex128, collapse on
 // qubit 29 (half the state read, all of it written): 4.08 ms (6.3 TB/s)
 // against 4.78 ms for the grid-stride k_collapse.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_collapse_streams(typename V2<R>::T *__restrict__ psi,
-                                                             uint64_t nchunks, int target,
-                                                             int keep, double sq) {
+                                                             uint64_t nchunks, uint64_t mask,
+                                                             uint64_t pat, double sq) {
   using T = typename V2<R>::T;
   constexpr int kItems = 4, S = 4;
   if (((uint64_t)blockIdx.x + 1) * (4 * kItems * S) > nchunks) return;
@@ -1403,7 +1405,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse_streams(typename V2<R>::T *
   }
 #pragma unroll
   for (int j = 0; j < kItems * S; ++j)
-    a[j] = (int)((idx[j] >> target) & 1ull) == keep ? ld<true>(psi + idx[j]) : czero<T>();
+    a[j] = (idx[j] & mask) == pat ? ld<true>(psi + idx[j]) : czero<T>();
 #pragma unroll
   for (int j = 0; j < kItems * S; ++j) {
     T r = a[j];
@@ -1414,14 +1416,21 @@ __global__ __launch_bounds__(kBlock) void k_collapse_streams(typename V2<R>::T *
 }
 
 template <typename R>
+hipError_t launch_collapse_pattern(void *psi, int n, uint64_t mask, uint64_t pat, double total,
+                                   hipStream_t s) {
+  const uint64_t nchunks = (1ull << n) >> 6;
+  if (n < 13) return hipErrorInvalidValue;  // the stream order needs 128 chunks
+  hipLaunchKernelGGL(k_collapse_streams<R>, dim3((unsigned)(nchunks / 64)), dim3(kBlock), 0, s,
+                     (typename V2<R>::T *)psi, nchunks, mask, pat, sqrt(total));
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_collapse(void *psi, int n, int target, int keep, double total, hipStream_t s) {
   const uint64_t len = 1ull << n;
   if (streaming(n, sizeof(typename V2<R>::T))) {
     // states above the last-level cache: the stream order
-    const uint64_t nchunks = len >> 6;
-    hipLaunchKernelGGL(k_collapse_streams<R>, dim3((unsigned)(nchunks / 64)), dim3(kBlock), 0, s,
-                       (typename V2<R>::T *)psi, nchunks, target, keep, sqrt(total));
-    return hipGetLastError();
+    return launch_collapse_pattern<R>(psi, n, 1ull << target, (uint64_t)keep << target, total, s);
   }
   unsigned grid = grid_for(len, kBlock * 4);
   if (grid > 262144) grid = 262144;
@@ -1817,6 +1826,8 @@ hipError_t launch_chunk_sums(const void *psi, int n, uint64_t chunk, double *d_o
   template hipError_t launch_prob_split<R>(const void *, int, int, double *, double *,          \
                                            hipStream_t);                                         \
   template hipError_t launch_collapse<R>(void *, int, int, int, double, hipStream_t);           \
+  template hipError_t launch_collapse_pattern<R>(void *, int, uint64_t, uint64_t, double,        \
+                                                 hipStream_t);                                   \
   template hipError_t launch_collapse_split<R>(void *, int, int, int, double, int, double *,     \
                                                double *, hipStream_t);                           \
   template hipError_t launch_measure_deferred<R>(void *, int, int, double, double *, double *,  \

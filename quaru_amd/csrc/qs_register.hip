@@ -694,6 +694,10 @@ __global__ void k_collapse_mask(typename std::conditional<std::is_same<R, double
 template <typename R>
 hipError_t collapse_mask(qs_register *r, uint64_t mask, uint64_t pat, double total) {
   using T = typename std::conditional<std::is_same<R, double>::value, double2, float2>::type;
+  // states above the last-level cache (256 MiB): the stream order of
+  // launch_collapse_pattern
+  if (((uint64_t)sizeof(T) << r->n) > (256ull << 20))
+    return qs::launch_collapse_pattern<R>(r->psi, r->n, mask, pat, total, r->stream);
   uint64_t g = (r->len + 1023) / 1024;
   if (g > 262144) g = 262144;
   if (g < 1) g = 1;
